@@ -1,0 +1,261 @@
+"""Benchmark of the MI355X multiplicative BLD solver (bld_mult).
+
+Metric: S-element updates/s = sum over instances of x*y*z*iterations / wall
+time, for the batched FP32 solver, beside the plain FP64 form, plus solve
+latency percentiles under serving (--serving).
+
+Headline workload (one "step" = one full batched solve from S0):
+  256 independent instances, x, y ~ U{32..512}, z ~ U{4..32}, X sampled
+  from the allocation model with a ~ U[10, 80], b = 1, alpha = 1, beta = 1,
+  200 iterations each, FP32 bulk with psi_appr. Inputs are resident in HBM
+  before the timed region; S restarts from the resident S0 every step.
+
+Also measured (same JSON line):
+  * fp64_plain_same_workload: the plain FP64 form (exact digamma) solving the
+    same 256 instances one at a time, and the ratio to the batched FP32 rate;
+  * config1_fp64: one 1024x1024x32 instance, 500 iterations, exact digamma;
+  * config2_fp32: one 2048x2048x64 instance, 300 iterations, psi_appr, with
+    its normwise error and log-marginal gap against the FP64 form;
+  * roofline of the fused pass kernel of the headline batch (HIP events on
+    the batch stream): algorithmic bytes 2*x*y*z*4 + x*y*4 per launch;
+  * cpu_baseline: the FP64 C++/OpenMP oracle (a port of the reference CPU
+    solver, which cannot be built here) on the reference's CPU config
+    100x100x17, 1000 iterations, exact digamma.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--serving]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bnmf-algs_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_c3(hip, n=256, seed=0xB1D0, iters=200):
+    rng = np.random.default_rng(seed)
+    insts = []
+    for t in range(n):
+        x, y = int(rng.integers(32, 513)), int(rng.integers(32, 513))
+        z = int(rng.integers(4, 33))
+        a = float(rng.uniform(10, 80))
+        p = hip.Params(a, 1.0, [1.0] * x, [1.0] * z)
+        X = hip.sample_X(seed + 17 * t, (x, y, z), p)
+        S0 = hip.init_S(seed + 17 * t + 1, X, z)
+        insts.append(hip.Instance(X=X, z=z, params=p, S0=S0))
+    return insts
+
+
+def single(hip, x, y, z, a, seed):
+    p = hip.Params(a, 1.0, [1.0] * x, [1.0] * z)
+    X = hip.sample_X(seed, (x, y, z), p)
+    S0 = hip.init_S(seed + 1, X, z)
+    return hip.Instance(X=X, z=z, params=p, S0=S0)
+
+
+def as_dtype(inst, hip, dt):
+    return hip.Instance(X=inst.X.astype(dt), z=inst.z, params=inst.params,
+                        S0=inst.S0.astype(dt), seed=inst.seed)
+
+
+def timed_solves(b, iters, steps, warmup):
+    for _ in range(warmup):
+        b.solve(iters)
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.solve_async(iters)
+    b.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def cpu_baseline(hip):
+    from oracle import oracle
+    inst = single(hip, 100, 100, 17, 40.0, 0xB1D0)
+    iters = 1000
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.bld_mult(inst.X, inst.z, inst.params.alpha, inst.params.beta, inst.S0,
+                        max_iter=iters)
+        times.append(time.perf_counter() - t0)
+    med = sorted(times)[1]
+    upd = 100 * 100 * 17 * iters
+    return {"value": upd / med, "unit": "S-element updates/s", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": "reference CPU config: X 100x100 (allocation model a=40, b=1, alpha=1, "
+                      "beta=1), z=17, 1000 iterations, exact digamma; median of 3 runs "
+                      f"({med:.2f} s each); FP64 C++/OpenMP oracle"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--serving", action="store_true", help="also run the serving workload")
+    ap.add_argument("--requests", type=int, default=10000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-loop", action="store_true")
+    ap.add_argument("--only-headline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(instances would shard across GPUs with no communication).", file=sys.stderr)
+        sys.exit(2)
+
+    import bnmf_hip as hip
+    hip.init(0)
+    out = {}
+
+    # ----------------------------------------------------- headline: config 3
+    log("[bench] building 256-instance batch")
+    c3 = make_c3(hip)
+    iters3 = 200
+    elems3 = sum(i.X.shape[0] * i.X.shape[1] * i.z for i in c3)
+    fibers3 = sum(i.X.shape[0] * i.X.shape[1] for i in c3)
+    c3f = [as_dtype(i, hip, np.float32) for i in c3]
+    b32 = hip.Batch(c3f, dtype=np.float32, use_psi_appr=True)
+    log("[bench] timing batched FP32 solves")
+    t_step = timed_solves(b32, iters3, args.steps, args.warmup)
+    value = elems3 * iters3 / t_step
+    S32 = [b32.S(n) for n in range(len(c3f))]
+    lm32 = b32.log_marginal()
+    b32.solve(iters3)  # restore S (log_marginal reuses the partial buffers)
+    pass_ms, iter_ms = b32.time_pass(50)
+    alg_bytes = 2 * elems3 * 4 + fibers3 * 4
+    achieved = alg_bytes / (pass_ms * 1e-3) / 1e9
+    info = b32.info()
+    log(f"[bench] batched FP32: {t_step*1e3:.2f} ms/solve, {value:.3e} upd/s, "
+        f"pass {pass_ms*1e3:.1f} us ({achieved:.0f} GB/s), iter {iter_ms*1e3:.1f} us")
+
+    # ------------------------------------- plain FP64, same instances, one by one
+    fp64 = None
+    if not args.no_fp64_loop and not args.only_headline:
+        log("[bench] plain FP64 loop over the same 256 instances")
+        c3d = [as_dtype(i, hip, np.float64) for i in c3]
+        singles = [hip.Batch([i], dtype=np.float64, use_psi_appr=False) for i in c3d]
+        for s in singles[:4]:
+            s.solve(2)
+        t0 = time.perf_counter()
+        for s in singles:
+            s.solve(iters3)
+        t64 = time.perf_counter() - t0
+        errs, gaps = [], []
+        for n, s in enumerate(singles):
+            S64 = s.S(0)
+            errs.append(rel_err(S32[n], S64))
+        lm64 = np.array([s.log_marginal()[0] for s in singles])
+        gaps = np.abs(lm32 - lm64) / np.abs(lm64)
+        for s in singles:
+            s.close()
+        fp64 = {"value": elems3 * iters3 / t64, "unit": "S-element updates/s",
+                "seconds": t64, "dtype": "f64", "digamma": "exact",
+                "mode": "one instance per solve call, 256 calls in sequence"}
+        out["fp64_plain_same_workload"] = fp64
+        out["ratio_vs_fp64_plain"] = value / fp64["value"]
+        out["accuracy_vs_fp64"] = {"max_rel_err_S_normwise": float(max(errs)),
+                                   "max_log_marginal_gap": float(gaps.max()),
+                                   "instances": len(errs)}
+        log(f"[bench] FP64 plain loop: {t64:.2f} s, {fp64['value']:.3e} upd/s; "
+            f"max rel err {max(errs):.2e}, max lm gap {gaps.max():.2e}")
+
+    # --------------------------------------------------------- config 1 (FP64)
+    if not args.only_headline:
+        log("[bench] config 1: 1024x1024x32 FP64 exact, 500 iterations")
+        c1 = single(hip, 1024, 1024, 32, 40.0, 0xB1D1)
+        b1 = hip.Batch([c1], dtype=np.float64, use_psi_appr=False)
+        t1 = timed_solves(b1, 500, 1, 1)
+        p1, it1 = b1.time_pass(20)
+        e1 = 1024 * 1024 * 32
+        out["config1_fp64"] = {
+            "value": e1 * 500 / t1, "unit": "S-element updates/s", "ms_per_solve": t1 * 1e3,
+            "pass_us": p1 * 1e3, "iter_us": it1 * 1e3,
+            "pass_GBps": (2 * e1 * 8 + 1024 * 1024 * 8) / (p1 * 1e-3) / 1e9}
+        b1.close()
+        log(f"[bench] config1: {t1*1e3:.1f} ms, pass {p1*1e3:.1f} us")
+
+        log("[bench] config 2: 2048x2048x64 FP32 psi_appr, 300 iterations")
+        c2 = single(hip, 2048, 2048, 64, 40.0, 0xB1D2)
+        b2 = hip.Batch([as_dtype(c2, hip, np.float32)], dtype=np.float32, use_psi_appr=True)
+        t2 = timed_solves(b2, 300, 1, 1)
+        S2 = b2.S(0)
+        lm2 = b2.log_marginal()[0]
+        b2.solve(300)
+        p2, it2 = b2.time_pass(20)
+        b2.close()
+        b2d = hip.Batch([c2], dtype=np.float64, use_psi_appr=True)
+        b2d.solve(300)
+        S2d = b2d.S(0)
+        lm2d = b2d.log_marginal()[0]
+        b2d.close()
+        e2 = 2048 * 2048 * 64
+        out["config2_fp32"] = {
+            "value": e2 * 300 / t2, "unit": "S-element updates/s", "ms_per_solve": t2 * 1e3,
+            "pass_us": p2 * 1e3, "iter_us": it2 * 1e3,
+            "pass_GBps": (2 * e2 * 4 + 2048 * 2048 * 4) / (p2 * 1e-3) / 1e9,
+            "rel_err_S_vs_fp64": rel_err(S2, S2d),
+            "log_marginal_gap_vs_fp64": abs(lm2 - lm2d) / abs(lm2d)}
+        del S2, S2d
+        log(f"[bench] config2: {t2*1e3:.1f} ms, pass {p2*1e3:.1f} us, "
+            f"err {out['config2_fp32']['rel_err_S_vs_fp64']:.2e}")
+
+    if args.serving:
+        import serving_bench
+        out["serving"] = serving_bench.run(hip, args.requests, log)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (FP64 oracle, reference CPU config)")
+        cpu = cpu_baseline(hip)
+
+    line = {
+        "metric": "bld_mult S-elem updates/s (FP32 batched vs FP64 plain) + p50/p99 solve latency",
+        "value": value,
+        "unit": "S-element updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (allocation-model samples, seeded)",
+        "config": {"workload": "batched FP32 bld_mult: 256 instances, x,y~U{32..512}, "
+                               "z~U{4..32}, a~U[10,80], b=1, alpha=beta=1, 200 iterations, "
+                               "psi_appr, S0 resident in HBM",
+                   "instances": len(c3), "elements": elems3, "iterations": iters3,
+                   "pass_items": info["pass_items"], "z_buckets": info["n_buckets"],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "bld_pass_kernel (FP32 update), all z buckets of one iteration",
+                     "pass_us": pass_ms * 1e3, "iter_us": iter_ms * 1e3,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,396 @@
+"""Python host mirror of the MI355X-native multiplicative BLD solver.
+
+Thin ctypes layer over lib/libbnmf_hip.so (C ABI: include/bnmf_hip.h). The
+names follow the reference's C++ API (eozd/bnmf-algs):
+
+  bld_mult / bld_mult_cuda   bld::bld_mult, bld::bld_mult_cuda
+                             (src/bld/bld_mult/bld_mult.hpp:83-86, :226-229)
+  Params                     alloc_model::Params
+                             (src/alloc_model/alloc_model_params.hpp:25-84)
+  log_marginal_S             alloc_model::log_marginal_S
+                             (src/alloc_model/alloc_model_funcs.hpp:574-586)
+  tensor_sums                cuda::tensor_sums (src/cuda/tensor_ops.hpp:40-42)
+  bnmf_priors / sample_S     alloc_model_funcs.hpp:463-552 (seeded)
+  init_S                     bld_mult_funcs.hpp:36-74 (seeded)
+
+Parameter errors raise ValueError (the reference throws
+std::invalid_argument under BUILD_TESTING, src/defs.hpp:12-16); device
+errors raise RuntimeError. Every solver call runs on the GPU; there is no
+CPU path, and importing this module fails if the library is not built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbnmf_hip.so")
+
+F32, F64 = 0, 1
+OK, ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE = 0, 1, 2, 3
+ERR_NOT_READY, ERR_NOT_FOUND, ERR_REJECTED, ERR_TIMEOUT = 4, 5, 6, 7
+
+_D = ctypes.POINTER(ctypes.c_double)
+_SZ = ctypes.c_size_t
+_VP = ctypes.c_void_p
+
+
+class bnmf_bld_desc(ctypes.Structure):
+    _fields_ = [("x", _SZ), ("y", _SZ), ("z", _SZ), ("X", _VP), ("alpha", _D), ("beta", _D),
+                ("a", ctypes.c_double), ("b", ctypes.c_double), ("S0", _VP),
+                ("seed", ctypes.c_uint64)]
+
+
+class bnmf_batch_info(ctypes.Structure):
+    _fields_ = [("n_instances", _SZ), ("elements", _SZ), ("fibers", _SZ),
+                ("device_bytes", _SZ), ("pass_items", _SZ), ("prep_items", _SZ),
+                ("n_buckets", ctypes.c_int)]
+
+
+class bnmf_request_stats(ctypes.Structure):
+    _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("iterations", _SZ), ("device_bytes", _SZ)]
+
+
+class bnmf_server_stats(ctypes.Structure):
+    _fields_ = [("submitted", _SZ), ("completed", _SZ), ("rejected", _SZ), ("active", _SZ),
+                ("queued", _SZ), ("steps", _SZ), ("bytes_in_use", _SZ), ("peak_bytes", _SZ),
+                ("peak_active", _SZ), ("busy_s", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libbnmf_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it (python -c 'import "
+                               f"__graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        st = ctypes.c_int
+        L.bnmf_last_error.restype = ctypes.c_char_p
+        L.bnmf_abi_version.restype = ctypes.c_int
+        L.bnmf_init.argtypes = [ctypes.c_int]
+        L.bnmf_init.restype = st
+        L.bnmf_bld_mult_f64.argtypes = [_VP, _SZ, _SZ, _SZ, _D, _D, _SZ, ctypes.c_int,
+                                        ctypes.c_double, _VP, ctypes.c_uint64, _VP]
+        L.bnmf_bld_mult_f64.restype = st
+        L.bnmf_bld_mult_f32.argtypes = L.bnmf_bld_mult_f64.argtypes
+        L.bnmf_bld_mult_f32.restype = st
+        L.bnmf_log_marginal_S.argtypes = [ctypes.c_int, _VP, _SZ, _SZ, _SZ, _D, _D,
+                                          ctypes.c_double, ctypes.c_double, _D]
+        L.bnmf_log_marginal_S.restype = st
+        L.bnmf_tensor_sums.argtypes = [ctypes.c_int, _VP, _SZ, _SZ, _SZ, _VP, _VP, _VP]
+        L.bnmf_tensor_sums.restype = st
+        L.bnmf_batch_create.argtypes = [ctypes.POINTER(_VP), ctypes.c_int,
+                                        ctypes.POINTER(bnmf_bld_desc), _SZ, ctypes.c_int,
+                                        ctypes.c_double]
+        L.bnmf_batch_create.restype = st
+        for name in ("bnmf_batch_solve", "bnmf_batch_solve_async"):
+            getattr(L, name).argtypes = [_VP, _SZ]
+            getattr(L, name).restype = st
+        L.bnmf_batch_synchronize.argtypes = [_VP]
+        L.bnmf_batch_synchronize.restype = st
+        L.bnmf_batch_get_S.argtypes = [_VP, _SZ, _VP]
+        L.bnmf_batch_get_S.restype = st
+        L.bnmf_batch_log_marginal.argtypes = [_VP, _D]
+        L.bnmf_batch_log_marginal.restype = st
+        L.bnmf_batch_time_pass.argtypes = [_VP, ctypes.c_int, _D, _D]
+        L.bnmf_batch_time_pass.restype = st
+        L.bnmf_batch_get_info.argtypes = [_VP, ctypes.POINTER(bnmf_batch_info)]
+        L.bnmf_batch_get_info.restype = st
+        L.bnmf_batch_destroy.argtypes = [_VP]
+        L.bnmf_batch_destroy.restype = None
+        L.bnmf_sample_X.argtypes = [ctypes.c_uint64, _SZ, _SZ, _SZ, _D, _D, ctypes.c_double,
+                                    ctypes.c_double, _D]
+        L.bnmf_sample_X.restype = st
+        L.bnmf_sample_S.argtypes = L.bnmf_sample_X.argtypes
+        L.bnmf_sample_S.restype = st
+        L.bnmf_init_S.argtypes = [ctypes.c_int, ctypes.c_uint64, _VP, _SZ, _SZ, _SZ, _VP]
+        L.bnmf_init_S.restype = st
+        L.bnmf_psi_eval.argtypes = [ctypes.c_int, _D, _D, _SZ]
+        L.bnmf_psi_eval.restype = st
+        if hasattr(L, "bnmf_server_create"):
+            L.bnmf_server_create.argtypes = [ctypes.POINTER(_VP), ctypes.c_int, _SZ, _SZ,
+                                             ctypes.c_int, ctypes.c_double]
+            L.bnmf_server_create.restype = st
+            L.bnmf_server_submit.argtypes = [_VP, ctypes.POINTER(bnmf_bld_desc), _SZ,
+                                             ctypes.POINTER(ctypes.c_uint64)]
+            L.bnmf_server_submit.restype = st
+            L.bnmf_server_poll.argtypes = [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
+            L.bnmf_server_poll.restype = st
+            L.bnmf_server_wait.argtypes = [_VP, ctypes.c_uint64, ctypes.c_double]
+            L.bnmf_server_wait.restype = st
+            L.bnmf_server_collect.argtypes = [_VP, ctypes.c_uint64, _VP, _D,
+                                              ctypes.POINTER(bnmf_request_stats)]
+            L.bnmf_server_collect.restype = st
+            L.bnmf_server_get_stats.argtypes = [_VP, ctypes.POINTER(bnmf_server_stats)]
+            L.bnmf_server_get_stats.restype = st
+            L.bnmf_server_request_bytes.argtypes = [ctypes.c_int, _SZ, _SZ, _SZ]
+            L.bnmf_server_request_bytes.restype = _SZ
+            L.bnmf_server_destroy.argtypes = [_VP]
+            L.bnmf_server_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(status: int, what: str):
+    if status == OK:
+        return
+    msg = lib().bnmf_last_error().decode()
+    if status == ERR_INVALID_ARGUMENT:
+        raise ValueError(f"{what}: {msg}")
+    if status == ERR_OUT_OF_MEMORY:
+        raise MemoryError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: status {status}: {msg}")
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def _np_dtype(dtype) -> np.dtype:
+    dt = np.dtype(dtype)
+    if dt not in (np.dtype(np.float32), np.dtype(np.float64)):
+        raise ValueError("dtype must be float32 or float64")
+    return dt
+
+
+def _code(dt: np.dtype) -> int:
+    return F32 if dt == np.float32 else F64
+
+
+@dataclass
+class Params:
+    """alloc_model::Params: Gamma(a, b) prior on L, Dirichlet(alpha) on W's
+    columns (len x), Dirichlet(beta) on H's columns (len z)."""
+    a: float
+    b: float
+    alpha: Sequence[float]
+    beta: Sequence[float]
+
+    @classmethod
+    def default(cls, shape):
+        """Params(shape) (alloc_model_params.hpp:78-81): a=1, b=10, ones."""
+        x, _, z = shape
+        return cls(1.0, 10.0, [1.0] * x, [1.0] * z)
+
+
+def init(device: int = 0):
+    _check(lib().bnmf_init(int(device)), "bnmf_init")
+
+
+def _check_bld_params(X: np.ndarray, z: int, params: Params):
+    # check_bld_params (src/bld/util_details.hpp:20-28)
+    if X.ndim != 2:
+        raise ValueError("X must be a matrix")
+    if len(params.alpha) != X.shape[0]:
+        raise ValueError("Number of alpha parameters must be equal to number of rows of X")
+    if len(params.beta) != z:
+        raise ValueError("Number of beta parameters must be equal to z")
+
+
+def bld_mult(X, z: int, model_params: Params, max_iter: int = 1000,
+             use_psi_appr: bool = False, eps: float = 1e-50, *, S0=None, seed: int = 0,
+             dtype=np.float64) -> np.ndarray:
+    """bld::bld_mult: S (x, y, z) with S_ij+ ~= X after max_iter
+    multiplicative updates, on the GPU. dtype float64 runs the plain FP64
+    form, float32 the FP32-bulk form. S0 overrides the seeded init_S."""
+    dt = _np_dtype(dtype)
+    X = np.ascontiguousarray(X, dtype=dt)
+    _check_bld_params(X, z, model_params)
+    x, y = X.shape
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    S0a = None
+    if S0 is not None:
+        S0a = np.ascontiguousarray(S0, dtype=dt)
+        if S0a.shape != (x, y, z):
+            raise ValueError("S0 must have shape (x, y, z)")
+    S = np.empty((x, y, z), dtype=dt)
+    fn = lib().bnmf_bld_mult_f32 if dt == np.float32 else lib().bnmf_bld_mult_f64
+    _check(fn(_ptr(X), x, y, z, _dptr(alpha), _dptr(beta), int(max_iter), int(bool(use_psi_appr)),
+              float(eps), _ptr(S0a), int(seed) & (2**64 - 1), _ptr(S)), "bld_mult")
+    return S
+
+
+# The reference's GPU driver has the same signature; both run on the GPU here.
+bld_mult_cuda = bld_mult
+
+
+def log_marginal_S(S, model_params: Params) -> float:
+    """alloc_model::log_marginal_S in FP64 on the GPU."""
+    S = np.asarray(S)
+    dt = np.dtype(np.float32) if S.dtype == np.float32 else np.dtype(np.float64)
+    S = np.ascontiguousarray(S, dtype=dt)
+    x, y, z = S.shape
+    if len(model_params.alpha) != x:
+        raise ValueError("Number of alpha parameters must be equal to S.dimension(0)")
+    if len(model_params.beta) != z:
+        raise ValueError("Number of beta parameters must be equal to S.dimension(2)")
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    out = np.zeros(1)
+    _check(lib().bnmf_log_marginal_S(_code(dt), _ptr(S), x, y, z, _dptr(alpha), _dptr(beta),
+                                     float(model_params.a), float(model_params.b), _dptr(out)),
+           "log_marginal_S")
+    return float(out[0])
+
+
+def tensor_sums(S):
+    """cuda::tensor_sums: (S_pjk (y,z), S_ipk (x,z), S_ijp (x,y))."""
+    S = np.asarray(S)
+    dt = np.dtype(np.float32) if S.dtype == np.float32 else np.dtype(np.float64)
+    S = np.ascontiguousarray(S, dtype=dt)
+    x, y, z = S.shape
+    pjk, ipk, ijp = (np.empty((y, z), dt), np.empty((x, z), dt), np.empty((x, y), dt))
+    _check(lib().bnmf_tensor_sums(_code(dt), _ptr(S), x, y, z, _ptr(pjk), _ptr(ipk), _ptr(ijp)),
+           "tensor_sums")
+    return pjk, ipk, ijp
+
+
+def sample_X(seed: int, shape, model_params: Params) -> np.ndarray:
+    """X = S_ij+ of an allocation-model sample (bnmf_priors + sample_S)."""
+    x, y, z = shape
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    X = np.empty((x, y))
+    _check(lib().bnmf_sample_X(int(seed), x, y, z, _dptr(alpha), _dptr(beta),
+                               float(model_params.a), float(model_params.b), _dptr(X)), "sample_X")
+    return X
+
+
+def sample_S(seed: int, shape, model_params: Params) -> np.ndarray:
+    x, y, z = shape
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    S = np.empty((x, y, z))
+    _check(lib().bnmf_sample_S(int(seed), x, y, z, _dptr(alpha), _dptr(beta),
+                               float(model_params.a), float(model_params.b), _dptr(S)), "sample_S")
+    return S
+
+
+def init_S(seed: int, X, z: int, dtype=np.float64) -> np.ndarray:
+    dt = _np_dtype(dtype)
+    X = np.ascontiguousarray(X, dtype=dt)
+    x, y = X.shape
+    S = np.empty((x, y, z), dtype=dt)
+    _check(lib().bnmf_init_S(_code(dt), int(seed), _ptr(X), x, y, z, _ptr(S)), "init_S")
+    return S
+
+
+def psi_eval(kind: int, x) -> np.ndarray:
+    """Device digamma forms (0: FP32 bulk, 1: exact FP64, 2: psi_appr FP64,
+    3: FP32 psi(s+1) hot-path form evaluated at x = s+1)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.empty_like(x)
+    _check(lib().bnmf_psi_eval(int(kind), _dptr(x), _dptr(out), x.size), "psi_eval")
+    return out
+
+
+@dataclass
+class Instance:
+    """One BLD problem of a batch: X (x, y), depth z, model params, and an
+    optional S0 (else drawn as init_S with `seed`)."""
+    X: np.ndarray
+    z: int
+    params: Params
+    S0: Optional[np.ndarray] = None
+    seed: int = 0
+    _keep: list = field(default_factory=list, repr=False)
+
+
+def _descs(instances: Sequence[Instance], dt: np.dtype):
+    arr = (bnmf_bld_desc * max(1, len(instances)))()
+    keep = []
+    for n, inst in enumerate(instances):
+        X = np.ascontiguousarray(inst.X, dtype=dt)
+        _check_bld_params(X, inst.z, inst.params)
+        alpha = np.ascontiguousarray(inst.params.alpha, dtype=np.float64)
+        beta = np.ascontiguousarray(inst.params.beta, dtype=np.float64)
+        S0 = None if inst.S0 is None else np.ascontiguousarray(inst.S0, dtype=dt)
+        if S0 is not None and S0.shape != (X.shape[0], X.shape[1], inst.z):
+            raise ValueError("S0 must have shape (x, y, z)")
+        keep += [X, alpha, beta, S0]
+        d = arr[n]
+        d.x, d.y, d.z = X.shape[0], X.shape[1], inst.z
+        d.X = X.ctypes.data
+        d.alpha = _dptr(alpha)
+        d.beta = _dptr(beta)
+        d.a, d.b = float(inst.params.a), float(inst.params.b)
+        d.S0 = None if S0 is None else S0.ctypes.data
+        d.seed = int(inst.seed) & (2**64 - 1)
+    return arr, keep
+
+
+class Batch:
+    """Many independent instances solved together on the GPU (batched fused
+    pass over a device work list, per-iteration chain replayed from a
+    hipGraph). Inputs are uploaded once; S stays resident in HBM."""
+
+    def __init__(self, instances: Sequence[Instance], dtype=np.float32,
+                 use_psi_appr: bool = True, eps: float = 1e-50):
+        self.dt = _np_dtype(dtype)
+        self.shapes = [(np.shape(i.X)[0], np.shape(i.X)[1], i.z) for i in instances]
+        arr, keep = _descs(instances, self.dt)
+        h = _VP()
+        _check(lib().bnmf_batch_create(ctypes.byref(h), _code(self.dt), arr, len(instances),
+                                       int(bool(use_psi_appr)), float(eps)), "batch_create")
+        del keep
+        self._h = h
+
+    def solve(self, max_iter: int):
+        _check(lib().bnmf_batch_solve(self._h, int(max_iter)), "batch_solve")
+
+    def solve_async(self, max_iter: int):
+        _check(lib().bnmf_batch_solve_async(self._h, int(max_iter)), "batch_solve_async")
+
+    def synchronize(self):
+        _check(lib().bnmf_batch_synchronize(self._h), "batch_synchronize")
+
+    def S(self, idx: int) -> np.ndarray:
+        out = np.empty(self.shapes[idx], dtype=self.dt)
+        _check(lib().bnmf_batch_get_S(self._h, int(idx), _ptr(out)), "batch_get_S")
+        return out
+
+    def log_marginal(self) -> np.ndarray:
+        out = np.zeros(max(1, len(self.shapes)))
+        _check(lib().bnmf_batch_log_marginal(self._h, _dptr(out)), "batch_log_marginal")
+        return out[:len(self.shapes)]
+
+    def time_pass(self, reps: int = 20):
+        p, it = np.zeros(1), np.zeros(1)
+        _check(lib().bnmf_batch_time_pass(self._h, int(reps), _dptr(p), _dptr(it)), "time_pass")
+        return float(p[0]), float(it[0])
+
+    def info(self) -> dict:
+        inf = bnmf_batch_info()
+        _check(lib().bnmf_batch_get_info(self._h, ctypes.byref(inf)), "batch_get_info")
+        return {k: getattr(inf, k) for k, _ in bnmf_batch_info._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bnmf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,451 @@
+// bld_batch.cpp — planning, HBM layout and launch sequencing of the BLD
+// solver (host side of the fused kernels in bld_kernels.hip).
+//
+// The reference driver (src/bld/bld_mult/bld_mult.hpp:226-324) allocates a
+// pitched 3D buffer per tensor, copies the axis sums to the host every
+// iteration and launches 7 kernels on 4 streams per iteration. Here the whole
+// iteration stays on the device: one arena holds every instance's tensors,
+// each iteration is `prep` + one fused `pass` per z bucket, and the chain is
+// captured once into a hipGraph and replayed.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "bnmf_internal.hpp"
+
+namespace bnmf {
+
+namespace {
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+}  // namespace
+
+// --------------------------------------------------------------- runtime
+void ensure_runtime() {
+    static std::once_flag once;
+    static hipError_t status = hipSuccess;
+    std::call_once(once, [] { status = init_kernel_attributes(); });
+    BNMF_HIP(status);
+}
+
+DeviceBuffer::DeviceBuffer(size_t n) : bytes(n) {
+    if (n) BNMF_HIP(hipMalloc(&ptr, n));
+}
+DeviceBuffer::~DeviceBuffer() {
+    if (ptr) (void)hipFree(ptr);
+}
+DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+        if (ptr) (void)hipFree(ptr);
+        ptr = o.ptr;
+        bytes = o.bytes;
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    return *this;
+}
+
+// ---------------------------------------------------------------- planner
+uint32_t z_bucket(uint32_t z) {
+    if (z <= 8) return 8;
+    if (z <= 16) return 16;
+    if (z <= 32) return 32;
+    return 64;
+}
+
+uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
+    (void)dtype;
+    (void)y;
+    (void)z;
+    // Row-tile height trades the S_+jk partial traffic (1/TI of S) against
+    // the number of tiles available to fill 256 CUs.
+    if (x >= 1024) return 64;
+    if (x >= 256) return 32;
+    return 16;
+}
+
+InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z) {
+    InstLayout L{};
+    const size_t es = dtype == BNMF_F32 ? 4 : 8;
+    L.x = x;
+    L.y = y;
+    L.z = z;
+    L.zp = z | 1u;
+    L.zb = z_bucket(z);
+    L.ti_rows = choose_ti_rows(dtype, x, y, z);
+    L.n_ti = ceil_div(x, L.ti_rows);
+    L.n_tj = ceil_div(y, kTileCols);
+    L.nA = ceil_div(x, kPrepRows);
+    L.n_cj = ceil_div(y, kPrepCols);
+    size_t off = 0;
+    auto take = [&](size_t n) {
+        const size_t o = off;
+        off = align_up(off + n);
+        return o;
+    };
+    const size_t xyz = (size_t)x * y * z;
+    L.off_S = take(xyz * es);
+    L.off_S0 = take(xyz * es);
+    L.off_X = take((size_t)x * y * es);
+    L.off_alpha = take((size_t)x * 8);
+    L.off_beta = take((size_t)z * 8);
+    L.off_psib = take((size_t)y * z * es);
+    L.off_psia = take((size_t)x * z * es);
+    L.off_partA = take((size_t)L.nA * z * es);
+    L.off_pjk = take((size_t)L.n_ti * y * z * 8);  // double capacity (scoring)
+    L.off_ipk = take((size_t)L.n_tj * x * z * 8);
+    L.off_score = take(((size_t)L.n_ti * L.n_tj + L.nA + L.n_cj + (size_t)L.nA * z) * 8);
+    L.off_result = take(8);
+    L.bytes = off;
+    return L;
+}
+
+BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first) {
+    char* p = static_cast<char*>(base);
+    BldInst d{};
+    d.S = p + L.off_S;
+    d.S_src = first ? p + L.off_S0 : p + L.off_S;
+    d.X = p + L.off_X;
+    d.alpha = reinterpret_cast<const double*>(p + L.off_alpha);
+    d.beta = reinterpret_cast<const double*>(p + L.off_beta);
+    d.psib = p + L.off_psib;
+    d.psia = p + L.off_psia;
+    d.partA = p + L.off_partA;
+    d.part_pjk = p + L.off_pjk;
+    d.part_ipk = p + L.off_ipk;
+    d.score = reinterpret_cast<double*>(p + L.off_score);
+    d.result = reinterpret_cast<double*>(p + L.off_result);
+    d.sijp = nullptr;
+    d.eps = eps;
+    d.a = a;
+    d.b = b;
+    d.inv_z = 1.0f / (float)L.z;
+    d.x = L.x;
+    d.y = L.y;
+    d.z = L.z;
+    d.zp = L.zp;
+    d.ti_rows = L.ti_rows;
+    d.n_ti = L.n_ti;
+    d.n_tj = L.n_tj;
+    d.nA = L.nA;
+    d.n_cj = L.n_cj;
+    uint32_t zq = 1, lg = 0;
+    while (zq < L.z) {
+        zq <<= 1;
+        ++lg;
+    }
+    d.zq = zq;
+    d.zq_log2 = lg;
+    return d;
+}
+
+// ------------------------------------------------------------------ batch
+Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& specs)
+    : dtype_(dtype), psi_(psi_mode), eps_(eps), es_(dtype == BNMF_F32 ? 4 : 8), specs_(specs) {
+    ensure_runtime();
+    int dev = 0;
+    BNMF_HIP(hipGetDevice(&dev));
+    BNMF_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
+
+    // layouts and arena
+    std::vector<size_t> base_off(specs.size());
+    size_t total = 0;
+    for (size_t n = 0; n < specs.size(); ++n) {
+        const InstSpec& s = specs[n];
+        BNMF_REQUIRE(s.z >= 1 && s.z <= kMaxZ, "z must be in [1, 64]");
+        layouts_.push_back(plan_instance(dtype, s.x, s.y, s.z));
+        base_off[n] = total;
+        total += layouts_.back().bytes;
+    }
+    arena_ = DeviceBuffer(std::max<size_t>(total, kAlign));
+    device_bytes_ = arena_.bytes;
+
+    // uploads (X, alpha, beta, S0)
+    std::vector<char> tmp;
+    for (size_t n = 0; n < specs.size(); ++n) {
+        const InstSpec& s = specs[n];
+        const InstLayout& L = layouts_[n];
+        char* b = arena_.as<char>(base_off[n]);
+        const size_t xy = (size_t)s.x * s.y, xyz = L.elements();
+        if (xyz == 0) continue;
+        BNMF_HIP(hipMemcpy(b + L.off_X, s.X, xy * es_, hipMemcpyHostToDevice));
+        BNMF_HIP(hipMemcpy(b + L.off_alpha, s.alpha, (size_t)s.x * 8, hipMemcpyHostToDevice));
+        BNMF_HIP(hipMemcpy(b + L.off_beta, s.beta, (size_t)s.z * 8, hipMemcpyHostToDevice));
+        const void* S0 = s.S0;
+        if (!S0) {
+            tmp.resize(xyz * es_);
+            if (dtype == BNMF_F32)
+                init_S<float>(s.seed, static_cast<const float*>(s.X), s.x, s.y, s.z,
+                              reinterpret_cast<float*>(tmp.data()));
+            else
+                init_S<double>(s.seed, static_cast<const double*>(s.X), s.x, s.y, s.z,
+                               reinterpret_cast<double*>(tmp.data()));
+            S0 = tmp.data();
+        }
+        BNMF_HIP(hipMemcpy(b + L.off_S0, S0, xyz * es_, hipMemcpyHostToDevice));
+        BNMF_HIP(hipMemcpy(b + L.off_S, S0, xyz * es_, hipMemcpyHostToDevice));
+    }
+
+    // descriptors and work lists
+    std::vector<BldInst> d_first, d_main;
+    std::vector<PrepItem> prep;
+    std::vector<std::vector<PassItem>> pass_b(4);
+    std::vector<uint32_t> zp_b(4, 1);
+    auto bidx = [](uint32_t zb) { return zb == 8 ? 0 : zb == 16 ? 1 : zb == 32 ? 2 : 3; };
+    for (size_t n = 0; n < specs.size(); ++n) {
+        const InstLayout& L = layouts_[n];
+        char* b = arena_.as<char>(base_off[n]);
+        d_first.push_back(make_desc(L, b, eps, specs[n].a, specs[n].b, true));
+        d_main.push_back(make_desc(L, b, eps, specs[n].a, specs[n].b, false));
+        if (L.elements() == 0) continue;
+        const int bi = bidx(L.zb);
+        zp_b[bi] = std::max(zp_b[bi], L.zp);
+        for (uint32_t ti = 0; ti < L.n_ti; ++ti)
+            for (uint32_t tj = 0; tj < L.n_tj; ++tj)
+                pass_b[bi].push_back(PassItem{(uint32_t)n, ti, tj, 0});
+        for (uint32_t c = 0; c < L.n_cj; ++c) prep.push_back(PrepItem{(uint32_t)n, 0, c, 0});
+        for (uint32_t c = 0; c < L.nA; ++c) prep.push_back(PrepItem{(uint32_t)n, 1, c, 0});
+    }
+    size_t moff = 0;
+    auto mtake = [&](size_t nbytes) {
+        const size_t o = moff;
+        moff = align_up(moff + nbytes);
+        return o;
+    };
+    off_desc_first_ = mtake(d_first.size() * sizeof(BldInst));
+    off_desc_ = mtake(d_main.size() * sizeof(BldInst));
+    off_prep_items_ = mtake(prep.size() * sizeof(PrepItem));
+    off_prep_count_ = mtake(sizeof(uint32_t));
+    n_prep_items_ = (uint32_t)prep.size();
+    const int zbs[4] = {8, 16, 32, 64};
+    for (int bi = 0; bi < 4; ++bi) {
+        if (pass_b[bi].empty()) continue;
+        Bucket bk;
+        bk.zb = zbs[bi];
+        bk.zp_max = zp_b[bi];
+        bk.n_items = (uint32_t)pass_b[bi].size();
+        bk.off_items = mtake(pass_b[bi].size() * sizeof(PassItem));
+        bk.off_count = mtake(sizeof(uint32_t));
+        buckets_.push_back(bk);
+    }
+    meta_ = DeviceBuffer(std::max<size_t>(moff, kAlign));
+    device_bytes_ += meta_.bytes;
+    char* m = meta_.as<char>();
+    if (!d_first.empty()) {
+        BNMF_HIP(hipMemcpy(m + off_desc_first_, d_first.data(), d_first.size() * sizeof(BldInst), hipMemcpyHostToDevice));
+        BNMF_HIP(hipMemcpy(m + off_desc_, d_main.data(), d_main.size() * sizeof(BldInst), hipMemcpyHostToDevice));
+    }
+    if (!prep.empty())
+        BNMF_HIP(hipMemcpy(m + off_prep_items_, prep.data(), prep.size() * sizeof(PrepItem), hipMemcpyHostToDevice));
+    BNMF_HIP(hipMemcpy(m + off_prep_count_, &n_prep_items_, sizeof(uint32_t), hipMemcpyHostToDevice));
+    size_t bpos = 0;
+    for (int bi = 0; bi < 4; ++bi) {
+        if (pass_b[bi].empty()) continue;
+        Bucket& bk = buckets_[bpos++];
+        BNMF_HIP(hipMemcpy(m + bk.off_items, pass_b[bi].data(), pass_b[bi].size() * sizeof(PassItem), hipMemcpyHostToDevice));
+        BNMF_HIP(hipMemcpy(m + bk.off_count, &bk.n_items, sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    BNMF_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+Batch::~Batch() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph1_) (void)hipGraphExecDestroy(graph1_);
+    if (graphU_) (void)hipGraphExecDestroy(graphU_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::enqueue_pass(int mode, bool first) {
+    char* m = meta_.as<char>();
+    const BldInst* d = reinterpret_cast<const BldInst*>(m + (first ? off_desc_first_ : off_desc_));
+    for (const Bucket& bk : buckets_) {
+        const dim3 grid(bk.n_items);
+        BNMF_HIP(launch_pass(dtype_, psi_, mode, bk.zb, grid, stream_, d,
+                             reinterpret_cast<const PassItem*>(m + bk.off_items),
+                             reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zp_max));
+    }
+}
+
+void Batch::enqueue_iteration(bool first) {
+    if (n_prep_items_ == 0) return;
+    char* m = meta_.as<char>();
+    const BldInst* d = reinterpret_cast<const BldInst*>(m + off_desc_);
+    BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(n_prep_items_), stream_, d,
+                         reinterpret_cast<const PrepItem*>(m + off_prep_items_),
+                         reinterpret_cast<const uint32_t*>(m + off_prep_count_)));
+    enqueue_pass(MODE_UPDATE, first);
+}
+
+void Batch::capture_graphs() {
+    if (graph1_) return;
+    auto capture = [&](int iters, hipGraphExec_t* out) {
+        hipGraph_t g = nullptr;
+        BNMF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int t = 0; t < iters; ++t) enqueue_iteration(false);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        BNMF_HIP(hipStreamEndCapture(stream_, &g));
+        hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        BNMF_HIP(e);
+    };
+    capture(1, &graph1_);
+    capture(unroll_, &graphU_);
+}
+
+void Batch::solve_async(size_t iters) {
+    if (n_prep_items_ == 0) return;
+    if (iters == 0) {
+        // S = S0 for every instance
+        size_t base = 0;
+        for (const InstLayout& L : layouts_) {
+            char* b = arena_.as<char>(base);
+            BNMF_HIP(hipMemcpyAsync(b + L.off_S, b + L.off_S0, L.elements() * es_,
+                                    hipMemcpyDeviceToDevice, stream_));
+            base += L.bytes;
+        }
+        return;
+    }
+    // partial axis sums of S0, then the first iteration reading S0
+    enqueue_pass(MODE_SUMS, true);
+    enqueue_iteration(true);
+    size_t rem = iters - 1;
+    if (rem) capture_graphs();
+    while (rem >= (size_t)unroll_) {
+        BNMF_HIP(hipGraphLaunch(graphU_, stream_));
+        rem -= unroll_;
+    }
+    while (rem > 0) {
+        BNMF_HIP(hipGraphLaunch(graph1_, stream_));
+        --rem;
+    }
+}
+
+void Batch::synchronize() { BNMF_HIP(hipStreamSynchronize(stream_)); }
+
+void Batch::get_S(size_t idx, void* host) const {
+    BNMF_REQUIRE(idx < layouts_.size(), "instance index out of range");
+    size_t base = 0;
+    for (size_t n = 0; n < idx; ++n) base += layouts_[n].bytes;
+    const InstLayout& L = layouts_[idx];
+    if (L.elements() == 0) return;
+    BNMF_HIP(hipStreamSynchronize(stream_));
+    BNMF_HIP(hipMemcpy(host, arena_.as<char>(base) + L.off_S, L.elements() * es_, hipMemcpyDeviceToHost));
+}
+
+void Batch::log_marginal(double* out) {
+    if (n_prep_items_ > 0) {
+        enqueue_pass(MODE_SCORE, false);
+        char* m = meta_.as<char>();
+        BNMF_HIP(launch_score(dim3(n_prep_items_), stream_, reinterpret_cast<const BldInst*>(m + off_desc_),
+                              reinterpret_cast<const PrepItem*>(m + off_prep_items_),
+                              reinterpret_cast<const uint32_t*>(m + off_prep_count_),
+                              (uint32_t)layouts_.size()));
+    }
+    BNMF_HIP(hipStreamSynchronize(stream_));
+    size_t base = 0;
+    for (size_t n = 0; n < layouts_.size(); ++n) {
+        const InstLayout& L = layouts_[n];
+        if (L.elements() == 0) {
+            out[n] = 0.0;
+        } else {
+            BNMF_HIP(hipMemcpy(&out[n], arena_.as<char>(base) + L.off_result, 8, hipMemcpyDeviceToHost));
+        }
+        base += L.bytes;
+    }
+}
+
+void Batch::time_pass(int reps, double* pass_ms, double* iter_ms) {
+    BNMF_REQUIRE(reps > 0, "reps must be positive");
+    if (n_prep_items_ == 0) {
+        *pass_ms = *iter_ms = 0.0;
+        return;
+    }
+    std::vector<hipEvent_t> ea(reps), eb(reps);
+    hipEvent_t e0, e1;
+    BNMF_HIP(hipEventCreate(&e0));
+    BNMF_HIP(hipEventCreate(&e1));
+    for (int r = 0; r < reps; ++r) {
+        BNMF_HIP(hipEventCreate(&ea[r]));
+        BNMF_HIP(hipEventCreate(&eb[r]));
+    }
+    char* m = meta_.as<char>();
+    const BldInst* d = reinterpret_cast<const BldInst*>(m + off_desc_);
+    BNMF_HIP(hipEventRecord(e0, stream_));
+    for (int r = 0; r < reps; ++r) {
+        BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(n_prep_items_), stream_, d,
+                             reinterpret_cast<const PrepItem*>(m + off_prep_items_),
+                             reinterpret_cast<const uint32_t*>(m + off_prep_count_)));
+        BNMF_HIP(hipEventRecord(ea[r], stream_));
+        enqueue_pass(MODE_UPDATE, false);
+        BNMF_HIP(hipEventRecord(eb[r], stream_));
+    }
+    BNMF_HIP(hipEventRecord(e1, stream_));
+    BNMF_HIP(hipEventSynchronize(e1));
+    double tot = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        float ms = 0.f;
+        BNMF_HIP(hipEventElapsedTime(&ms, ea[r], eb[r]));
+        tot += ms;
+    }
+    float all = 0.f;
+    BNMF_HIP(hipEventElapsedTime(&all, e0, e1));
+    *pass_ms = tot / reps;
+    *iter_ms = (double)all / reps;
+    for (int r = 0; r < reps; ++r) {
+        (void)hipEventDestroy(ea[r]);
+        (void)hipEventDestroy(eb[r]);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+void Batch::tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp) {
+    BNMF_REQUIRE(layouts_.size() == 1, "tensor_sums works on a single instance");
+    const InstLayout& L = layouts_[0];
+    if (L.elements() == 0) return;
+    // sums of S (the current tensor), fiber sums into a scratch buffer
+    DeviceBuffer sijp((size_t)L.x * L.y * es_);
+    BldInst d = make_desc(L, arena_.ptr, eps_, 1.0, 1.0, false);
+    d.sijp = sijp.ptr;
+    DeviceBuffer dd(sizeof(BldInst));
+    BNMF_HIP(hipMemcpy(dd.ptr, &d, sizeof(BldInst), hipMemcpyHostToDevice));
+    char* m = meta_.as<char>();
+    const Bucket& bk = buckets_.at(0);
+    BNMF_HIP(launch_pass(dtype_, psi_, MODE_SUMS, bk.zb, dim3(bk.n_items), stream_, dd.as<BldInst>(),
+                         reinterpret_cast<const PassItem*>(m + bk.off_items),
+                         reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zp_max));
+    BNMF_HIP(launch_prep(dtype_, psi_, true, dim3(n_prep_items_), stream_, dd.as<BldInst>(),
+                         reinterpret_cast<const PrepItem*>(m + off_prep_items_),
+                         reinterpret_cast<const uint32_t*>(m + off_prep_count_)));
+    BNMF_HIP(hipStreamSynchronize(stream_));
+    if (S_pjk) BNMF_HIP(hipMemcpy(S_pjk, arena_.as<char>(L.off_psib), (size_t)L.y * L.z * es_, hipMemcpyDeviceToHost));
+    if (S_ipk) BNMF_HIP(hipMemcpy(S_ipk, arena_.as<char>(L.off_psia), (size_t)L.x * L.z * es_, hipMemcpyDeviceToHost));
+    if (S_ijp) BNMF_HIP(hipMemcpy(S_ijp, sijp.ptr, (size_t)L.x * L.y * es_, hipMemcpyDeviceToHost));
+}
+
+bnmf_batch_info Batch::info() const {
+    bnmf_batch_info in{};
+    in.n_instances = layouts_.size();
+    for (const InstLayout& L : layouts_) {
+        in.elements += L.elements();
+        in.fibers += (size_t)L.x * L.y;
+    }
+    in.device_bytes = device_bytes_;
+    for (const Bucket& bk : buckets_) in.pass_items += bk.n_items;
+    in.prep_items = n_prep_items_;
+    in.n_buckets = (int)buckets_.size();
+    return in;
+}
+
+}  // namespace bnmf

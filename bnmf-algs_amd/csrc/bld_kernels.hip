@@ -1,0 +1,705 @@
+// bld_kernels.hip — CDNA4 (gfx950) kernels of the multiplicative BLD solver.
+//
+// One solver iteration of the reference (src/bld/bld_mult/bld_mult.hpp:123-147)
+// is two launches here:
+//
+//  prep  (small, O((x+y)z)): finishes the axis sums of the current S from
+//        per-tile partials, adds the Dirichlet parameters and evaluates
+//        psi(beta_eph) (update_beta_eph + the psi half of update_grad_plus,
+//        bld_mult_funcs.hpp:152-163,177-192) and psi(alpha_eph) together with
+//        column partials of alpha_eph (update_alpha_eph + update_grad_minus,
+//        :130-141,205-229).
+//  pass  (the hot path, one read and one write of S): per fiber (i,j) it
+//        forms grad_plus in registers, the fiber sums S_ij+, nom_ij and
+//        denom_ij (update_nom_mult / update_denom_mult, :242-290), applies
+//        update_S (:307-327), and accumulates the partial axis sums S_+jk and
+//        S_i+k of the NEW tensor for the next iteration's prep.
+//
+// Pass mapping: a work item is a tile of TI rows x 64 columns of one
+// instance. A block of WAVES wavefronts shares the tile's columns; wave w
+// walks rows i0+w, i0+w+WAVES, ... For each row the 64 fibers of the tile are
+// one contiguous run of 64*z elements of S: lanes load it coalesced (element
+// e = c*64 + lane), scatter it into a wave-private LDS buffer with an odd
+// fiber stride zp (bank-conflict free), then each lane owns one fiber and
+// runs the z-long fiber reductions sequentially in registers, so no
+// cross-lane reduction is needed for S_ij+, nom or denom. The updated fiber
+// goes back through LDS and leaves with coalesced stores. Row sums over the
+// 64 fibers (S_i+k partial) are LDS column sums; column sums over the rows
+// (S_+jk partial) stay in registers across the row loop. The next row's
+// loads are issued before the current row is computed.
+//
+// All cross-block reductions are two-stage (per-tile partials, then a
+// fixed-order sum in prep), never atomics, so every result is bitwise
+// reproducible and independent of which instances share a launch.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bld_types.hpp"
+#include "digamma.hpp"
+
+namespace bnmf {
+
+enum PassMode : int { PASS_UPDATE = 0, PASS_SUMS = 1, PASS_SCORE = 2 };
+
+// Element index within a run of whole fibers -> fiber index. Exact for
+// e < 2^16 and z <= 64: (e + 0.5)/z lies at least 0.5/z from an integer and
+// the float error is below 1e-5.
+__device__ __forceinline__ uint32_t fiber_of(uint32_t e, float inv_z) {
+    return static_cast<uint32_t>((static_cast<float>(e) + 0.5f) * inv_z);
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename A>
+struct Kahan {
+    A s = A(0), c = A(0);
+    __device__ __forceinline__ void add(A v) {
+        const A yv = v - c;
+        const A t = s + yv;
+        c = (t - s) - yv;
+        s = t;
+    }
+};
+
+// Compensated sums only where the summand type is float; double sums stay
+// plain (the FP64 form follows the reference's plain sums).
+template <typename A>
+struct SumAcc;
+template <>
+struct SumAcc<float> {
+    Kahan<float> k;
+    __device__ __forceinline__ void add(float v) { k.add(v); }
+    __device__ __forceinline__ float get() const { return k.s; }
+};
+template <>
+struct SumAcc<double> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ double get() const { return s; }
+};
+
+template <typename T>
+__device__ __forceinline__ T recip_fast(T v);
+template <>
+__device__ __forceinline__ float recip_fast<float>(float v) { return __builtin_amdgcn_rcpf(v); }
+
+// Wavefronts per pass block: FP64 fibers of up to 64 elements use two waves
+// so that the block's LDS stays within the 160 KB of a CU.
+template <typename T, int ZB>
+__host__ __device__ constexpr int pass_waves() {
+    return (sizeof(T) == 8 && ZB == 64) ? 2 : 4;
+}
+
+// LDS bytes of the pass kernel for a given odd fiber stride.
+// Byte offset of the fiber buffers: after psib (64*zp), psiA (zp) and the
+// per-wave grad_minus rows (WAVES*zp), rounded up to 16 bytes.
+template <typename T, int WAVES>
+__host__ __device__ constexpr size_t pass_fib_offset(uint32_t zp) {
+    return ((size_t)(64 * zp + zp + WAVES * zp) * sizeof(T) + 15) / 16 * 16;
+}
+
+template <typename T, typename ACC, int WAVES>
+__host__ __device__ constexpr size_t pass_lds_bytes(uint32_t zp) {
+    return pass_fib_offset<T, WAVES>(zp) +
+           (size_t)WAVES * 64 * zp * (sizeof(ACC) > sizeof(T) ? sizeof(ACC) : sizeof(T));
+}
+
+// ---------------------------------------------------------------------------
+// pass kernel
+// ---------------------------------------------------------------------------
+template <typename T, typename ACC, int ZB, int MODE, int PSI, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+    bld_pass_kernel(const BldInst* __restrict__ insts, const PassItem* __restrict__ items,
+                    const uint32_t* __restrict__ n_items_p) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    constexpr uint32_t NT = 64 * WAVES;
+    constexpr size_t SLOT = sizeof(ACC) > sizeof(T) ? sizeof(ACC) : sizeof(T);
+    const uint32_t n_items = *n_items_p;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t w = tid >> 6;
+
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const PassItem item = items[it];
+        const BldInst* __restrict__ I = insts + item.inst;
+        const uint32_t x = I->x, y = I->y, z = I->z, zp = I->zp;
+        const float inv_z = I->inv_z;
+        const uint32_t i0 = item.ti * I->ti_rows;
+        const uint32_t i1 = min(x, i0 + I->ti_rows);
+        const uint32_t j0 = item.tj * kTileCols;
+        const uint32_t nj = min(kTileCols, y - j0);
+        const uint32_t n_el = nj * z;
+
+        T* const psib_l = reinterpret_cast<T*>(lds_raw);
+        T* const psiA_l = psib_l + 64 * zp;
+        T* const gm_l = psiA_l + zp + w * zp;
+        unsigned char* const fib_bytes = lds_raw + pass_fib_offset<T, WAVES>(zp);
+        T* const fib = reinterpret_cast<T*>(fib_bytes + (size_t)w * 64 * zp * SLOT);
+
+        const T* __restrict__ Ssrc = static_cast<const T*>(I->S_src);
+        T* __restrict__ Sdst = static_cast<T*>(I->S);
+        const double eps = I->eps;
+
+        if constexpr (MODE == PASS_UPDATE) {
+            const T* __restrict__ psib_g = static_cast<const T*>(I->psib) + (size_t)j0 * z;
+            for (uint32_t e = tid; e < n_el; e += NT) {
+                const uint32_t f = fiber_of(e, inv_z);
+                psib_l[f * zp + (e - f * z)] = psib_g[e];
+            }
+            if (tid < z) {
+                // A_k = sum_i alpha_eph(i,k), the argument of psi in
+                // update_grad_minus (bld_mult_funcs.hpp:213-220)
+                const T* __restrict__ pA = static_cast<const T*>(I->partA);
+                SumAcc<T> A;
+                for (uint32_t c = 0; c < I->nA; ++c) A.add(pA[(size_t)c * z + tid]);
+                psiA_l[tid] = psi_sel<PSI, T>(A.get());
+            }
+            __syncthreads();
+        }
+
+        ACC acc[ZB];
+#pragma unroll
+        for (int k = 0; k < ZB; ++k) acc[k] = ACC(0);
+        double lg_sum = 0.0;
+
+        const uint32_t kq = lane & (I->zq - 1u);
+        const uint32_t part = lane >> I->zq_log2;
+        const uint32_t L = 64u >> I->zq_log2;
+
+        T pf[ZB];
+        uint32_t i = i0 + w;
+        if (i < i1) {
+            const T* __restrict__ src = Ssrc + ((size_t)i * y + j0) * z;
+#pragma unroll
+            for (int c = 0; c < ZB; ++c) {
+                const uint32_t e = (uint32_t)c * 64u + lane;
+                if ((uint32_t)c * 64u < n_el) pf[c] = e < n_el ? src[e] : T(0);
+            }
+        }
+        for (; i < i1; i += WAVES) {
+            // scatter the current row into the wave's fiber buffer
+#pragma unroll
+            for (int c = 0; c < ZB; ++c) {
+                const uint32_t e = (uint32_t)c * 64u + lane;
+                if ((uint32_t)c * 64u < n_el && e < n_el) {
+                    const uint32_t f = fiber_of(e, inv_z);
+                    fib[f * zp + (e - f * z)] = pf[c];
+                }
+            }
+            // issue the next row's loads
+            const uint32_t inext = i + WAVES;
+            if (inext < i1) {
+                const T* __restrict__ src = Ssrc + ((size_t)inext * y + j0) * z;
+#pragma unroll
+                for (int c = 0; c < ZB; ++c) {
+                    const uint32_t e = (uint32_t)c * 64u + lane;
+                    if ((uint32_t)c * 64u < n_el) pf[c] = e < n_el ? src[e] : T(0);
+                }
+            }
+
+            if constexpr (MODE == PASS_UPDATE) {
+                const T* __restrict__ psia_g = static_cast<const T*>(I->psia) + (size_t)i * z;
+                if (lane < z) gm_l[lane] = psiA_l[lane] - psia_g[lane];
+                T xv = T(0);
+                if (lane < nj) xv = static_cast<const T*>(I->X)[(size_t)i * y + j0 + lane];
+                wave_sync();
+                if (lane < nj) {
+                    T* const my = fib + lane * zp;
+                    const T* const pb = psib_l + lane * zp;
+                    T gp[ZB];
+                    T sumS = T(0), nom = T(0), den = T(0);
+                    if constexpr (sizeof(T) == 8) {
+                        // FP64 plain form: the reference's operations in its
+                        // order (X_reciprocal :95, nom :254, denom :284-285,
+                        // update :319-323).
+                        const T xr = T(1.0) / (xv + eps);
+#pragma unroll
+                        for (int k = 0; k < ZB; ++k) {
+                            if (k < (int)z) {
+                                const T s = my[k];
+                                const T g = pb[k] - psi_sp1<PSI, T>(s);
+                                gp[k] = g;
+                                sumS += s;
+                                nom += s * xr * gm_l[k];
+                                den += s * xr * g;
+                            }
+                        }
+                        const T x_over_s = xv / (sumS + eps);
+#pragma unroll
+                        for (int k = 0; k < ZB; ++k) {
+                            if (k < (int)z) {
+                                const T s = my[k];
+                                const T sn = s * ((gp[k] + nom) * x_over_s / (gm_l[k] + den + eps));
+                                my[k] = sn;
+                                acc[k] += ACC(sn);
+                            }
+                        }
+                    } else {
+                        // FP32 bulk. eps = 1e-50 is not representable in
+                        // float; the reference's double-promoted guard makes
+                        // an X_ij = 0 or all-zero fiber update to exactly 0,
+                        // which is taken explicitly here.
+                        const float epsf = static_cast<float>(eps);
+                        const bool live = xv > 0.0f;
+                        const float xr = live ? 1.0f / (xv + epsf) : 0.0f;
+#pragma unroll
+                        for (int k = 0; k < ZB; ++k) {
+                            if (k < (int)z) {
+                                const float s = my[k];
+                                const float g = pb[k] - psi_sp1<PSI, T>(s);
+                                gp[k] = g;
+                                sumS += s;
+                                const float t = s * xr;
+                                nom = fmaf(t, gm_l[k], nom);
+                                den = fmaf(t, g, den);
+                            }
+                        }
+                        const bool upd = live && sumS != 0.0f;
+                        const float x_over_s = upd ? xv / (sumS + epsf) : 0.0f;
+                        const float den_e = den + epsf;
+#pragma unroll
+                        for (int k = 0; k < ZB; ++k) {
+                            if (k < (int)z) {
+                                const float s = my[k];
+                                const float fct = (gp[k] + nom) * x_over_s *
+                                                  recip_fast<float>(gm_l[k] + den_e);
+                                const float sn = upd ? s * fct : 0.0f;
+                                my[k] = sn;
+                                acc[k] += ACC(sn);
+                            }
+                        }
+                    }
+                }
+            } else {
+                wave_sync();
+                if (lane < nj) {
+                    const T* const my = fib + lane * zp;
+                    ACC fs = ACC(0);
+#pragma unroll
+                    for (int k = 0; k < ZB; ++k) {
+                        if (k < (int)z) {
+                            const T s = my[k];
+                            acc[k] += ACC(s);
+                            fs += ACC(s);
+                            if constexpr (MODE == PASS_SCORE) lg_sum += lgamma((double)s + 1.0);
+                        }
+                    }
+                    if constexpr (MODE == PASS_SUMS)
+                        if (I->sijp != nullptr) static_cast<ACC*>(I->sijp)[(size_t)i * y + j0 + lane] = fs;
+                }
+            }
+            wave_sync();
+
+            // S_i+k partial over the tile's columns (fixed order)
+            {
+                ACC rs = ACC(0);
+                if (kq < z)
+                    for (uint32_t f = part; f < nj; f += L) rs += ACC(fib[f * zp + kq]);
+                for (uint32_t off = I->zq; off < 64u; off <<= 1) rs += __shfl_xor(rs, (int)off);
+                if (part == 0 && kq < z)
+                    static_cast<ACC*>(I->part_ipk)[((size_t)item.tj * x + i) * z + kq] = rs;
+            }
+
+            if constexpr (MODE == PASS_UPDATE) {
+                T* __restrict__ dst = Sdst + ((size_t)i * y + j0) * z;
+#pragma unroll
+                for (int c = 0; c < ZB; ++c) {
+                    const uint32_t e = (uint32_t)c * 64u + lane;
+                    if ((uint32_t)c * 64u < n_el && e < n_el) {
+                        const uint32_t f = fiber_of(e, inv_z);
+                        dst[e] = fib[f * zp + (e - f * z)];
+                    }
+                }
+            }
+            wave_sync();
+        }
+
+        // S_+jk partial over the tile's rows: combine the waves in order
+        __syncthreads();
+        ACC* const comb_self = reinterpret_cast<ACC*>(fib_bytes + (size_t)w * 64 * zp * SLOT);
+        if (lane < nj) {
+#pragma unroll
+            for (int k = 0; k < ZB; ++k)
+                if (k < (int)z) comb_self[lane * zp + k] = acc[k];
+        }
+        __syncthreads();
+        {
+            ACC* __restrict__ pp = static_cast<ACC*>(I->part_pjk) + ((size_t)item.ti * y + j0) * z;
+            for (uint32_t e = tid; e < n_el; e += NT) {
+                const uint32_t f = fiber_of(e, inv_z);
+                const uint32_t o = f * zp + (e - f * z);
+                ACC s = ACC(0);
+#pragma unroll
+                for (int w2 = 0; w2 < WAVES; ++w2)
+                    s += reinterpret_cast<const ACC*>(fib_bytes + (size_t)w2 * 64 * zp * SLOT)[o];
+                pp[e] = s;
+            }
+        }
+        if constexpr (MODE == PASS_SCORE) {
+            // block sum of the lgamma(S+1) lane sums, fixed tree order
+            __syncthreads();
+            double* red = reinterpret_cast<double*>(fib_bytes);
+            for (uint32_t off = 32; off > 0; off >>= 1) lg_sum += __shfl_xor(lg_sum, (int)off);
+            if (lane == 0) red[w] = lg_sum;
+            __syncthreads();
+            if (tid == 0) {
+                double s = 0.0;
+                for (int w2 = 0; w2 < WAVES; ++w2) s += red[w2];
+                I->score[(size_t)item.ti * I->n_tj + item.tj] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// prep kernel
+// ---------------------------------------------------------------------------
+template <typename T, int PSI, bool RAW>
+__global__ void __launch_bounds__(256)
+    bld_prep_kernel(const BldInst* __restrict__ insts, const PrepItem* __restrict__ items,
+                    const uint32_t* __restrict__ n_items_p) {
+    __shared__ T aeph[kPrepRows * 64];
+    const uint32_t n_items = *n_items_p;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const PrepItem item = items[it];
+        const BldInst* __restrict__ I = insts + item.inst;
+        const uint32_t x = I->x, y = I->y, z = I->z;
+        const float inv_z = I->inv_z;
+        if (item.kind == 0) {
+            const uint32_t c0 = item.chunk * kPrepCols;
+            const uint32_t nc = min(kPrepCols, y - c0);
+            const T* __restrict__ pp = static_cast<const T*>(I->part_pjk);
+            T* __restrict__ psib = static_cast<T*>(I->psib);
+            const size_t stride = (size_t)y * z;
+            for (uint32_t e = tid; e < nc * z; e += 256) {
+                const size_t idx = (size_t)c0 * z + e;
+                const uint32_t k = e - fiber_of(e, inv_z) * z;
+                SumAcc<T> s;
+                for (uint32_t t = 0; t < I->n_ti; ++t) s.add(pp[t * stride + idx]);
+                if constexpr (RAW) {
+                    psib[idx] = s.get();
+                } else {
+                    const T beta_eph = s.get() + static_cast<T>(I->beta[k]);
+                    psib[idx] = psi_sel<PSI, T>(beta_eph);
+                }
+            }
+        } else {
+            const uint32_t r0 = item.chunk * kPrepRows;
+            const uint32_t nr = min(kPrepRows, x - r0);
+            const T* __restrict__ pp = static_cast<const T*>(I->part_ipk);
+            T* __restrict__ psia = static_cast<T*>(I->psia);
+            const size_t stride = (size_t)x * z;
+            for (uint32_t e = tid; e < nr * z; e += 256) {
+                const size_t idx = (size_t)r0 * z + e;
+                const uint32_t r = fiber_of(e, inv_z);
+                SumAcc<T> s;
+                for (uint32_t t = 0; t < I->n_tj; ++t) s.add(pp[t * stride + idx]);
+                if constexpr (RAW) {
+                    psia[idx] = s.get();
+                } else {
+                    const T alpha_eph = s.get() + static_cast<T>(I->alpha[r0 + r]);
+                    psia[idx] = psi_sel<PSI, T>(alpha_eph);
+                    aeph[e] = alpha_eph;
+                }
+            }
+            __syncthreads();
+            if (!RAW && tid < z) {
+                SumAcc<T> A;
+                for (uint32_t r = 0; r < nr; ++r) A.add(aeph[r * z + tid]);
+                static_cast<T*>(I->partA)[(size_t)item.chunk * z + tid] = A.get();
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// log-marginal scoring (alloc_model_funcs.hpp:574-586), FP64 throughout.
+// Needs a PASS_SCORE pass first: double partials of S_+jk / S_i+k and the
+// per-tile sums of lgamma(S_ijk + 1) (fourth term, :179-194).
+// score buffer layout: [n_ti*n_tj tile lgamma sums | nA row-chunk values |
+//                       n_cj column-chunk values | nA*z row-chunk column sums]
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum_256(double v, double* red) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t off = 32; off > 0; off >>= 1) v += __shfl_xor(v, (int)off);
+    __syncthreads();
+    if ((tid & 63u) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0) s = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return s;
+}
+
+__global__ void __launch_bounds__(256)
+    bld_score_terms_kernel(const BldInst* __restrict__ insts, const PrepItem* __restrict__ items,
+                           const uint32_t* __restrict__ n_items_p) {
+    __shared__ double red[4];
+    __shared__ double sik[kPrepRows * 64];
+    const uint32_t n_items = *n_items_p;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const PrepItem item = items[it];
+        const BldInst* __restrict__ I = insts + item.inst;
+        const uint32_t x = I->x, y = I->y, z = I->z;
+        const float inv_z = I->inv_z;
+        const size_t ntile = (size_t)I->n_ti * I->n_tj;
+        if (item.kind == 0) {
+            // second and third terms for a chunk of columns j
+            const uint32_t c0 = item.chunk * kPrepCols;
+            const uint32_t nc = min(kPrepCols, y - c0);
+            const double* __restrict__ pp = static_cast<const double*>(I->part_pjk);
+            const size_t stride = (size_t)y * z;
+            double beta_sum = 0.0;
+            for (uint32_t k = 0; k < z; ++k) beta_sum += I->beta[k];
+            double v = 0.0;
+            if (tid < nc) {
+                const uint32_t j = c0 + tid;
+                double spj = 0.0;
+                for (uint32_t k = 0; k < z; ++k) {
+                    double s = 0.0;
+                    for (uint32_t t = 0; t < I->n_ti; ++t) s += pp[t * stride + (size_t)j * z + k];
+                    spj += s;
+                    v += lgamma(I->beta[k] + s);
+                }
+                v -= lgamma(beta_sum + spj);
+                v += lgamma(I->a + spj);
+                v -= (I->a + spj) * log(I->b + 1.0);
+            }
+            const double s = block_sum_256(v, red);
+            if (tid == 0) I->score[ntile + I->nA + item.chunk] = s;
+        } else {
+            // first term for a chunk of rows i: sum lgamma(alpha_i + S_i+k)
+            // and the chunk's column sums of S_i+k (for S_++k)
+            const uint32_t r0 = item.chunk * kPrepRows;
+            const uint32_t nr = min(kPrepRows, x - r0);
+            const double* __restrict__ pp = static_cast<const double*>(I->part_ipk);
+            const size_t stride = (size_t)x * z;
+            double v = 0.0;
+            for (uint32_t e = tid; e < nr * z; e += 256) {
+                const size_t idx = (size_t)r0 * z + e;
+                const uint32_t r = fiber_of(e, inv_z);
+                double s = 0.0;
+                for (uint32_t t = 0; t < I->n_tj; ++t) s += pp[t * stride + idx];
+                sik[e] = s;
+                v += lgamma(I->alpha[r0 + r] + s);
+            }
+            const double s = block_sum_256(v, red);
+            if (tid == 0) I->score[ntile + item.chunk] = s;
+            if (tid < z) {
+                double cs = 0.0;
+                for (uint32_t r = 0; r < nr; ++r) cs += sik[r * z + tid];
+                I->score[ntile + I->nA + I->n_cj + (size_t)item.chunk * z + tid] = cs;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One block per instance: combines the partial terms in a fixed order.
+__global__ void __launch_bounds__(256)
+    bld_score_final_kernel(const BldInst* __restrict__ insts, uint32_t n_inst) {
+    __shared__ double red[4];
+    __shared__ double s_alpha_sum;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t in = blockIdx.x; in < n_inst; in += gridDim.x) {
+        const BldInst* __restrict__ I = insts + in;
+        if (I->result == nullptr) continue;
+        const uint32_t x = I->x, y = I->y, z = I->z;
+        const size_t ntile = (size_t)I->n_ti * I->n_tj;
+        const double* sc = I->score;
+        // alpha sums
+        double asum = 0.0, alg = 0.0;
+        for (uint32_t i = tid; i < x; i += 256) {
+            asum += I->alpha[i];
+            alg += lgamma(I->alpha[i]);
+        }
+        const double alpha_sum = block_sum_256(asum, red);
+        const double lg_alpha = block_sum_256(alg, red);
+        // per-k: S_++k from row-chunk column sums
+        double tk = 0.0;
+        if (tid == 0) s_alpha_sum = alpha_sum;
+        __syncthreads();
+        for (uint32_t k = tid; k < z; k += 256) {
+            double spk = 0.0;
+            for (uint32_t c = 0; c < I->nA; ++c) spk += sc[ntile + I->nA + I->n_cj + (size_t)c * z + k];
+            tk -= lgamma(s_alpha_sum + spk);
+        }
+        const double t_k = block_sum_256(tk, red);
+        double tl = 0.0;
+        for (size_t t = tid; t < ntile; t += 256) tl += sc[t];
+        const double fourth = block_sum_256(tl, red);
+        double tr = 0.0;
+        for (uint32_t c = tid; c < I->nA; c += 256) tr += sc[ntile + c];
+        const double rows = block_sum_256(tr, red);
+        double tc = 0.0;
+        for (uint32_t c = tid; c < I->n_cj; c += 256) tc += sc[ntile + I->nA + c];
+        const double cols = block_sum_256(tc, red);
+        if (tid == 0) {
+            double beta_sum = 0.0, lg_beta = 0.0;
+            for (uint32_t k = 0; k < z; ++k) {
+                beta_sum += I->beta[k];
+                lg_beta += lgamma(I->beta[k]);
+            }
+            const double first = (lgamma(alpha_sum) * z - lg_alpha * z) + (t_k + rows);
+            const double second_third = (lgamma(beta_sum) * y - lg_beta * y) +
+                                        (-lgamma(I->a) * y + I->a * log(I->b) * y) + cols;
+            *I->result = first + second_third - fourth;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (explicit instantiation table)
+// ---------------------------------------------------------------------------
+template <typename T, typename ACC, int ZB, int MODE, int PSI>
+static hipError_t launch_pass_t(dim3 grid, hipStream_t st, const BldInst* insts,
+                                const PassItem* items, const uint32_t* n_items, uint32_t zp) {
+    constexpr int WAVES = pass_waves<T, ZB>();
+    const size_t lds = pass_lds_bytes<T, ACC, WAVES>(zp);
+    auto* fn = bld_pass_kernel<T, ACC, ZB, MODE, PSI, WAVES>;
+    hipLaunchKernelGGL(fn, grid, dim3(64 * WAVES), lds, st, insts, items, n_items);
+    return hipGetLastError();
+}
+
+template <typename T, typename ACC, int MODE, int PSI>
+static hipError_t launch_pass_zb(int zb, dim3 grid, hipStream_t st, const BldInst* insts,
+                                 const PassItem* items, const uint32_t* n, uint32_t zp) {
+    switch (zb) {
+        case 8: return launch_pass_t<T, ACC, 8, MODE, PSI>(grid, st, insts, items, n, zp);
+        case 16: return launch_pass_t<T, ACC, 16, MODE, PSI>(grid, st, insts, items, n, zp);
+        case 32: return launch_pass_t<T, ACC, 32, MODE, PSI>(grid, st, insts, items, n, zp);
+        case 64: return launch_pass_t<T, ACC, 64, MODE, PSI>(grid, st, insts, items, n, zp);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// dtype: 0 = float, 1 = double. zp: the largest LDS fiber stride among the
+// instances in the launch (sizes the dynamic LDS).
+hipError_t launch_pass(int dtype, int psi, int mode, int zb, dim3 grid, hipStream_t st,
+                       const BldInst* insts, const PassItem* items, const uint32_t* n_items,
+                       uint32_t zp) {
+    if (dtype == 0) {
+        switch (mode) {
+            case PASS_UPDATE:
+                return launch_pass_zb<float, float, PASS_UPDATE, PSI_F32>(zb, grid, st, insts, items, n_items, zp);
+            case PASS_SUMS:
+                return launch_pass_zb<float, float, PASS_SUMS, PSI_F32>(zb, grid, st, insts, items, n_items, zp);
+            case PASS_SCORE:
+                return launch_pass_zb<float, double, PASS_SCORE, PSI_F32>(zb, grid, st, insts, items, n_items, zp);
+        }
+    } else {
+        switch (mode) {
+            case PASS_UPDATE:
+                if (psi == PSI_F64_APPR)
+                    return launch_pass_zb<double, double, PASS_UPDATE, PSI_F64_APPR>(zb, grid, st, insts, items, n_items, zp);
+                return launch_pass_zb<double, double, PASS_UPDATE, PSI_F64_EXACT>(zb, grid, st, insts, items, n_items, zp);
+            case PASS_SUMS:
+                return launch_pass_zb<double, double, PASS_SUMS, PSI_F64_EXACT>(zb, grid, st, insts, items, n_items, zp);
+            case PASS_SCORE:
+                return launch_pass_zb<double, double, PASS_SCORE, PSI_F64_EXACT>(zb, grid, st, insts, items, n_items, zp);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+size_t pass_lds(int dtype, int mode, uint32_t zp) {
+    if (dtype == 0)
+        return mode == PASS_SCORE ? pass_lds_bytes<float, double, 4>(zp) : pass_lds_bytes<float, float, 4>(zp);
+    return zp > 33 ? pass_lds_bytes<double, double, 2>(zp) : pass_lds_bytes<double, double, 4>(zp);
+}
+
+hipError_t launch_prep(int dtype, int psi, bool raw, dim3 grid, hipStream_t st,
+                       const BldInst* insts, const PrepItem* items, const uint32_t* n_items) {
+    if (dtype == 0) {
+        if (raw)
+            hipLaunchKernelGGL((bld_prep_kernel<float, PSI_F32, true>), grid, dim3(256), 0, st, insts, items, n_items);
+        else
+            hipLaunchKernelGGL((bld_prep_kernel<float, PSI_F32, false>), grid, dim3(256), 0, st, insts, items, n_items);
+    } else if (raw) {
+        hipLaunchKernelGGL((bld_prep_kernel<double, PSI_F64_EXACT, true>), grid, dim3(256), 0, st, insts, items, n_items);
+    } else if (psi == PSI_F64_APPR) {
+        hipLaunchKernelGGL((bld_prep_kernel<double, PSI_F64_APPR, false>), grid, dim3(256), 0, st, insts, items, n_items);
+    } else {
+        hipLaunchKernelGGL((bld_prep_kernel<double, PSI_F64_EXACT, false>), grid, dim3(256), 0, st, insts, items, n_items);
+    }
+    return hipGetLastError();
+}
+
+// Element-wise evaluation of the device digamma forms (test hook).
+__global__ void __launch_bounds__(256)
+    bld_psi_eval_kernel(int kind, const double* __restrict__ xin, double* __restrict__ out, size_t n) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const double v = xin[t];
+    double r;
+    if (kind == 0) {
+        const float f = (float)v;
+        r = (double)psi_f32(f);
+    } else if (kind == 3) {
+        r = (double)psi_sp1<PSI_F32, float>((float)v - 1.0f);
+    } else if (kind == 1) {
+        r = psi_f64(v);
+    } else {
+        r = psi_appr_ref<double>(v);
+    }
+    out[t] = r;
+}
+
+hipError_t launch_psi_eval(int kind, const double* xin, double* out, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(bld_psi_eval_kernel, dim3(blocks), dim3(256), 0, st, kind, xin, out, n);
+    return hipGetLastError();
+}
+
+// Sets the dynamic-LDS limit of every pass instantiation once per process.
+template <typename T, typename ACC, int MODE, int PSI>
+static hipError_t attr_zb() {
+    hipError_t e = hipSuccess;
+#define BNMF_ATTR(ZBV)                                                                          \
+    if (e == hipSuccess)                                                                        \
+        e = hipFuncSetAttribute(                                                                \
+            reinterpret_cast<const void*>(bld_pass_kernel<T, ACC, ZBV, MODE, PSI, pass_waves<T, ZBV>()>), \
+            hipFuncAttributeMaxDynamicSharedMemorySize,                                         \
+            (int)pass_lds_bytes<T, ACC, pass_waves<T, ZBV>()>(ZBV + 1));
+    BNMF_ATTR(8)
+    BNMF_ATTR(16)
+    BNMF_ATTR(32)
+    BNMF_ATTR(64)
+#undef BNMF_ATTR
+    return e;
+}
+
+hipError_t init_kernel_attributes() {
+    hipError_t e = attr_zb<float, float, PASS_UPDATE, PSI_F32>();
+    if (e == hipSuccess) e = attr_zb<float, float, PASS_SUMS, PSI_F32>();
+    if (e == hipSuccess) e = attr_zb<float, double, PASS_SCORE, PSI_F32>();
+    if (e == hipSuccess) e = attr_zb<double, double, PASS_UPDATE, PSI_F64_EXACT>();
+    if (e == hipSuccess) e = attr_zb<double, double, PASS_UPDATE, PSI_F64_APPR>();
+    if (e == hipSuccess) e = attr_zb<double, double, PASS_SUMS, PSI_F64_EXACT>();
+    if (e == hipSuccess) e = attr_zb<double, double, PASS_SCORE, PSI_F64_EXACT>();
+    return e;
+}
+
+hipError_t launch_score(dim3 grid, hipStream_t st, const BldInst* insts, const PrepItem* items,
+                        const uint32_t* n_items, uint32_t n_inst) {
+    hipLaunchKernelGGL(bld_score_terms_kernel, grid, dim3(256), 0, st, insts, items, n_items);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bld_score_final_kernel, dim3(n_inst < 1024 ? n_inst : 1024), dim3(256), 0,
+                       st, insts, n_inst);
+    return hipGetLastError();
+}
+
+}  // namespace bnmf

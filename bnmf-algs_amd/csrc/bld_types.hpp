@@ -1,0 +1,67 @@
+// bld_types.hpp — device-side descriptors shared by the host planner and the
+// HIP kernels of the multiplicative BLD solver.
+//
+// HBM layout of one instance (all dense, row-major, k fastest, no per-fiber
+// pitch; the reference pads every z-fiber with cudaMalloc3D,
+// src/cuda/memory/device_memory_3d.hpp:55-65):
+//   S        T[x*y*z]        the solution tensor, updated in place
+//   S_src    T[x*y*z]        tensor read by the first pass (S0, or S itself)
+//   X        T[x*y]
+//   alpha    double[x], beta double[z]    model parameters (Params::alpha/beta)
+//   psib     T[y*z]          psi(S_+jk + beta_k)          (beta_eph, :152-163)
+//   psia     T[x*z]          psi(S_i+k + alpha_i)         (alpha_eph, :130-141)
+//   partA    ACC[nA*z]       column sums of alpha_eph over row chunks of 64
+//   part_pjk ACC[n_ti*y*z]   per-row-tile partial S_+jk
+//   part_ipk ACC[n_tj*x*z]   per-column-tile partial S_i+k
+//   score    double[n_ti*n_tj + nA + n_cj]  log-marginal partials
+#pragma once
+
+#include <cstdint>
+
+namespace bnmf {
+
+// Fibers (columns j) per tile: one wavefront lane per fiber.
+constexpr uint32_t kTileCols = 64;
+// Rows per prep chunk (alpha_eph column-sum granularity) and columns per
+// prep chunk.
+constexpr uint32_t kPrepRows = 64;
+constexpr uint32_t kPrepCols = 64;
+
+struct BldInst {
+    void* S;              // updated tensor (pass output)
+    const void* S_src;    // tensor read by the pass (== S except the first pass)
+    const void* X;
+    const double* alpha;
+    const double* beta;
+    void* psib;
+    void* psia;
+    void* partA;
+    void* part_pjk;
+    void* part_ipk;
+    double* score;        // score partials (log-marginal), may be null
+    double* result;       // log-marginal output (device), may be null
+    void* sijp;           // fiber sums S_ij+ (sums-only pass), may be null
+    double eps;
+    double a, b;          // Gamma prior (log-marginal only)
+    float inv_z;          // 1.0f / z (fiber index from element index)
+    uint32_t x, y, z, zp; // zp: LDS fiber stride (z rounded up to odd)
+    uint32_t ti_rows;     // rows per pass tile
+    uint32_t n_ti, n_tj;  // tiles along rows / columns
+    uint32_t nA;          // prep row chunks
+    uint32_t n_cj;        // prep column chunks
+    uint32_t zq;          // next power of two >= z
+    uint32_t zq_log2;
+    uint32_t pad0;
+};
+
+struct PassItem {
+    uint32_t inst, ti, tj, pad;
+};
+
+// kind 0: column chunk (psib / column score terms); kind 1: row chunk
+// (psia, partA / row score terms).
+struct PrepItem {
+    uint32_t inst, kind, chunk, pad;
+};
+
+}  // namespace bnmf

@@ -1,0 +1,154 @@
+// bnmf_internal.hpp — host-side internals shared by the library sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bnmf_hip.h"
+#include "bld_types.hpp"
+
+namespace bnmf {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+    bnmf_status status;
+    Error(bnmf_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+void set_last_error(const std::string& msg);
+void clear_last_error();
+
+#define BNMF_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t bnmf_e_ = (call);                                                    \
+        if (bnmf_e_ != hipSuccess)                                                      \
+            throw ::bnmf::Error(bnmf_e_ == hipErrorOutOfMemory ? BNMF_ERR_OUT_OF_MEMORY \
+                                                               : BNMF_ERR_DEVICE,       \
+                                std::string(#call) + ": " + hipGetErrorString(bnmf_e_)); \
+    } while (0)
+
+#define BNMF_REQUIRE(cond, msg)                                               \
+    do {                                                                      \
+        if (!(cond)) throw ::bnmf::Error(BNMF_ERR_INVALID_ARGUMENT, (msg)); \
+    } while (0)
+
+// ----------------------------------------------------------------- kernels
+enum : int { MODE_UPDATE = 0, MODE_SUMS = 1, MODE_SCORE = 2 };
+enum : int { PSIM_F32 = 0, PSIM_F64_EXACT = 1, PSIM_F64_APPR = 2 };
+hipError_t launch_pass(int dtype, int psi, int mode, int zb, dim3 grid, hipStream_t st,
+                       const BldInst* insts, const PassItem* items, const uint32_t* n_items,
+                       uint32_t zp);
+size_t pass_lds(int dtype, int mode, uint32_t zp);
+hipError_t launch_prep(int dtype, int psi, bool raw, dim3 grid, hipStream_t st,
+                       const BldInst* insts, const PrepItem* items, const uint32_t* n_items);
+hipError_t launch_score(dim3 grid, hipStream_t st, const BldInst* insts, const PrepItem* items,
+                        const uint32_t* n_items, uint32_t n_inst);
+hipError_t launch_psi_eval(int kind, const double* xin, double* out, size_t n, hipStream_t st);
+hipError_t init_kernel_attributes();
+void ensure_runtime();  // per-process kernel attributes, once
+
+// ----------------------------------------------------------------- sampler
+int sample_X(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+             const double* beta, double a, double b, double* X);
+int sample_S(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+             const double* beta, double a, double b, double* S);
+template <typename T>
+int init_S(uint64_t seed, const T* X, size_t x, size_t y, size_t z, T* S);
+
+// ----------------------------------------------------------- tile planner
+// Largest z handled by the fused pass (fiber registers are sized by bucket).
+constexpr uint32_t kMaxZ = 64;
+uint32_t z_bucket(uint32_t z);
+uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z);
+
+// --------------------------------------------------------- device buffers
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t n);
+    ~DeviceBuffer();
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
+    template <typename T>
+    T* as(size_t byte_off = 0) const {
+        return reinterpret_cast<T*>(static_cast<char*>(ptr) + byte_off);
+    }
+};
+
+// ----------------------------------------------------------------- batches
+struct InstSpec {
+    uint32_t x = 0, y = 0, z = 0;
+    const void* X = nullptr;       // host
+    const double* alpha = nullptr; // host
+    const double* beta = nullptr;  // host
+    double a = 1.0, b = 1.0;
+    const void* S0 = nullptr;      // host or null
+    uint64_t seed = 0;
+};
+
+// Byte layout of one instance inside an arena.
+struct InstLayout {
+    uint32_t x, y, z, zp, zb, ti_rows, n_ti, n_tj, nA, n_cj;
+    size_t off_S, off_S0, off_X, off_alpha, off_beta, off_psib, off_psia, off_partA, off_pjk,
+        off_ipk, off_score, off_result;
+    size_t bytes;  // total, 256-byte aligned
+    size_t elements() const { return (size_t)x * y * z; }
+};
+InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z);
+BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first);
+
+// A fixed set of instances sharing one arena, one stream and one set of
+// captured per-iteration graphs.
+class Batch {
+   public:
+    Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& specs);
+    ~Batch();
+    void solve_async(size_t iters);
+    void synchronize();
+    void get_S(size_t idx, void* host) const;
+    void log_marginal(double* out);
+    void time_pass(int reps, double* pass_ms, double* iter_ms);
+    void tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp);
+    bnmf_batch_info info() const;
+    size_t size() const { return layouts_.size(); }
+
+   private:
+    struct Bucket {
+        int zb;
+        uint32_t zp_max;
+        uint32_t n_items;
+        size_t off_items;  // in meta_
+        size_t off_count;
+    };
+    void enqueue_iteration(bool first);
+    void enqueue_pass(int mode, bool first);
+    void capture_graphs();
+
+    int dtype_, psi_;
+    double eps_;
+    size_t es_;
+    std::vector<InstLayout> layouts_;
+    std::vector<InstSpec> specs_;
+    DeviceBuffer arena_;
+    DeviceBuffer meta_;  // descriptors, work lists, counts
+    size_t off_desc_first_ = 0, off_desc_ = 0, off_prep_items_ = 0, off_prep_count_ = 0;
+    uint32_t n_prep_items_ = 0;
+    std::vector<Bucket> buckets_;
+    hipStream_t stream_ = nullptr;
+    hipGraphExec_t graph1_ = nullptr, graphU_ = nullptr;
+    int unroll_ = 8;
+    int cus_ = 256;
+    size_t device_bytes_ = 0;
+};
+
+}  // namespace bnmf

@@ -1,0 +1,247 @@
+// sampler.cpp — seeded input synthesis for the BLD solver (host code).
+//
+// Distributions follow the reference's allocation model
+// (src/alloc_model/alloc_model_funcs.hpp:463-552) and init_S
+// (src/bld/bld_mult/bld_mult_funcs.hpp:36-74); the reference draws from GSL's
+// taus generator, whose stream is not reproduced. Here every draw comes from
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11) keyed by the seed, with the counter naming the
+// variate, so results are identical for any thread count.
+//   Gamma(a >= 1): Marsaglia & Tsang (2000); a < 1: Gamma(a+1) * U^(1/a).
+//   Normal: Box-Muller.  Dirichlet: normalised Gammas.
+//   Poisson: Knuth's product method for mean < 12, otherwise the
+//   transformed rejection method PTRS (Hormann 1993).
+// gsl_ran_gamma(r, a, b) takes b as the SCALE; the same convention is kept.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "bnmf_internal.hpp"
+
+namespace bnmf {
+namespace {
+
+struct Philox {
+    uint32_t k0, k1;
+    uint32_t c0, c1, c2, c3;
+    uint32_t buf[4];
+    int pos = 4;
+
+    Philox(uint64_t seed, uint32_t stream, uint64_t index)
+        : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), c0(0), c1(stream),
+          c2((uint32_t)index), c3((uint32_t)(index >> 32)) {}
+
+    static inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+        const uint64_t p = (uint64_t)a * b;
+        hi = (uint32_t)(p >> 32);
+        lo = (uint32_t)p;
+    }
+    void block() {
+        uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, a = k0, b = k1;
+        for (int r = 0; r < 10; ++r) {
+            uint32_t hi0, lo0, hi1, lo1;
+            mulhilo(0xD2511F53u, x0, hi0, lo0);
+            mulhilo(0xCD9E8D57u, x2, hi1, lo1);
+            const uint32_t n0 = hi1 ^ x1 ^ a, n2 = hi0 ^ x3 ^ b;
+            x0 = n0;
+            x1 = lo1;
+            x2 = n2;
+            x3 = lo0;
+            a += 0x9E3779B9u;
+            b += 0xBB67AE85u;
+        }
+        buf[0] = x0;
+        buf[1] = x1;
+        buf[2] = x2;
+        buf[3] = x3;
+        ++c0;  // next block of this variate's stream
+        pos = 0;
+    }
+    uint32_t next32() {
+        if (pos == 4) block();
+        return buf[pos++];
+    }
+    // uniform on the open interval (0, 1), 53 random bits
+    double uniform() {
+        const uint64_t hi = next32(), lo = next32();
+        const uint64_t m = ((hi << 21) ^ (lo >> 11)) & ((1ull << 53) - 1);
+        return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+    }
+    double normal() {
+        const double u1 = uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+double gamma_draw(Philox& g, double shape, double scale) {
+    if (shape < 1.0) {
+        const double u = g.uniform();
+        return gamma_draw(g, shape + 1.0, scale) * std::pow(u, 1.0 / shape);
+    }
+    const double d = shape - 1.0 / 3.0, c = 1.0 / std::sqrt(9.0 * d);
+    for (;;) {
+        double xn, v;
+        do {
+            xn = g.normal();
+            v = 1.0 + c * xn;
+        } while (v <= 0.0);
+        v = v * v * v;
+        const double u = g.uniform();
+        if (u < 1.0 - 0.0331 * xn * xn * xn * xn) return scale * d * v;
+        if (std::log(u) < 0.5 * xn * xn + d * (1.0 - v + std::log(v))) return scale * d * v;
+    }
+}
+
+double poisson_draw(Philox& g, double mu) {
+    if (!(mu > 0.0)) return 0.0;
+    if (mu < 12.0) {
+        const double lim = std::exp(-mu);
+        double prod = g.uniform();
+        double k = 0;
+        while (prod > lim) {
+            prod *= g.uniform();
+            k += 1;
+        }
+        return k;
+    }
+    // PTRS
+    const double smu = std::sqrt(mu);
+    const double b = 0.931 + 2.53 * smu;
+    const double a = -0.059 + 0.02483 * b;
+    const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    const double log_mu = std::log(mu);
+    for (;;) {
+        const double U = g.uniform() - 0.5;
+        const double V = g.uniform();
+        const double us = 0.5 - std::fabs(U);
+        const double k = std::floor((2.0 * a / us + b) * U + mu + 0.43);
+        if (us >= 0.07 && V <= vr) return k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (std::log(V) + std::log(inv_alpha) - std::log(a / (us * us) + b) <=
+            -mu + k * log_mu - std::lgamma(k + 1.0))
+            return k;
+    }
+}
+
+// stream ids: one per kind of variate
+enum : uint32_t { ST_L = 1, ST_W = 2, ST_H = 3, ST_X = 4, ST_S = 5, ST_INIT = 6 };
+
+template <typename F>
+void parallel_for(size_t n, F&& f) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        for (size_t t = 0; t < n; ++t) f(t);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t chunk = (n + nt - 1) / nt;
+    for (unsigned w = 0; w < nt; ++w) {
+        const size_t b = w * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([b, e, &f] {
+            for (size_t t = b; t < e; ++t) f(t);
+        });
+    }
+    for (auto& t : th) t.join();
+}
+
+// priors of bnmf_priors (alloc_model_funcs.hpp:463-505)
+void priors(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+            const double* beta, double a, double b, std::vector<double>& W,
+            std::vector<double>& H, std::vector<double>& L) {
+    L.resize(y);
+    for (size_t j = 0; j < y; ++j) {
+        Philox g(seed, ST_L, j);
+        L[j] = gamma_draw(g, a, b);
+    }
+    W.assign(x * z, 0.0);  // x*z, columns ~ Dir(alpha)
+    parallel_for(z, [&](size_t k) {
+        Philox g(seed, ST_W, k);
+        double s = 0.0;
+        std::vector<double> v(x);
+        for (size_t i = 0; i < x; ++i) {
+            v[i] = gamma_draw(g, alpha[i], 1.0);
+            s += v[i];
+        }
+        for (size_t i = 0; i < x; ++i) W[i * z + k] = v[i] / s;
+    });
+    H.assign(z * y, 0.0);  // z*y, columns ~ Dir(beta)
+    parallel_for(y, [&](size_t j) {
+        Philox g(seed, ST_H, j);
+        double s = 0.0;
+        std::vector<double> v(z);
+        for (size_t k = 0; k < z; ++k) {
+            v[k] = gamma_draw(g, beta[k], 1.0);
+            s += v[k];
+        }
+        for (size_t k = 0; k < z; ++k) H[k * y + j] = v[k] / s;
+    });
+}
+
+}  // namespace
+
+int sample_X(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+             const double* beta, double a, double b, double* X) {
+    std::vector<double> W, H, L;
+    priors(seed, x, y, z, alpha, beta, a, b, W, H, L);
+    parallel_for(x, [&](size_t i) {
+        for (size_t j = 0; j < y; ++j) {
+            double lam = 0.0;
+            for (size_t k = 0; k < z; ++k) lam += W[i * z + k] * H[k * y + j];
+            Philox g(seed, ST_X, (uint64_t)i * y + j);
+            X[i * y + j] = poisson_draw(g, lam * L[j]);
+        }
+    });
+    return 0;
+}
+
+int sample_S(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+             const double* beta, double a, double b, double* S) {
+    std::vector<double> W, H, L;
+    priors(seed, x, y, z, alpha, beta, a, b, W, H, L);
+    parallel_for(x, [&](size_t i) {
+        for (size_t j = 0; j < y; ++j)
+            for (size_t k = 0; k < z; ++k) {
+                Philox g(seed, ST_S, ((uint64_t)i * y + j) * z + k);
+                S[(i * y + j) * z + k] = poisson_draw(g, W[i * z + k] * H[k * y + j] * L[j]);
+            }
+    });
+    return 0;
+}
+
+template <typename T>
+int init_S(uint64_t seed, const T* X, size_t x, size_t y, size_t z, T* S) {
+    parallel_for(x * y, [&](size_t f) {
+        T* s = S + f * z;
+        const double xv = (double)X[f];
+        if (xv == 0.0) {
+            for (size_t k = 0; k < z; ++k) s[k] = T(0);
+            return;
+        }
+        Philox g(seed, ST_INIT, f);
+        double tot = 0.0;
+        double v[256];
+        std::vector<double> big;
+        double* vp = v;
+        if (z > 256) {
+            big.resize(z);
+            vp = big.data();
+        }
+        for (size_t k = 0; k < z; ++k) {
+            vp[k] = -std::log(g.uniform());  // Gamma(1, 1)
+            tot += vp[k];
+        }
+        for (size_t k = 0; k < z; ++k) s[k] = (T)(xv * (vp[k] / tot));
+    });
+    return 0;
+}
+
+template int init_S<float>(uint64_t, const float*, size_t, size_t, size_t, float*);
+template int init_S<double>(uint64_t, const double*, size_t, size_t, size_t, double*);
+
+}  // namespace bnmf

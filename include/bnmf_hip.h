@@ -1,0 +1,152 @@
+/* bnmf_hip.h — C ABI of the MI355X-native multiplicative BLD solver.
+ *
+ * Every entry point takes plain pointers and sizes; tensors are dense
+ * row-major (x rows, y columns, z fastest), matching the reference's Eigen
+ * RowMajor types (src/defs.hpp:26-52). Functions return a bnmf_status; the
+ * message of the last failure on the calling thread is bnmf_last_error().
+ *
+ * Citations are to eozd/bnmf-algs, relative to the reference root.
+ */
+#ifndef BNMF_HIP_H
+#define BNMF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum bnmf_status {
+    BNMF_OK = 0,
+    BNMF_ERR_INVALID_ARGUMENT = 1, /* BNMF_ASSERT failures (src/defs.hpp:12-20) */
+    BNMF_ERR_OUT_OF_MEMORY = 2,
+    BNMF_ERR_DEVICE = 3,           /* a HIP runtime call failed */
+    BNMF_ERR_NOT_READY = 4,        /* serving: result not finished yet */
+    BNMF_ERR_NOT_FOUND = 5,        /* serving: unknown ticket */
+    BNMF_ERR_REJECTED = 6,         /* serving: request exceeds the HBM budget */
+    BNMF_ERR_TIMEOUT = 7           /* serving: wait timed out */
+} bnmf_status;
+
+typedef enum bnmf_dtype { BNMF_F32 = 0, BNMF_F64 = 1 } bnmf_dtype;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* bnmf_last_error(void);
+/* ABI version of this header (major * 100 + minor). */
+int bnmf_abi_version(void);
+/* Selects the HIP device used by this thread's later calls
+ * (role of cuda::init, src/cuda/util.hpp:15-18). */
+bnmf_status bnmf_init(int device);
+
+/* ------------------------------------------------------------------------
+ * Drop-in solver: bld::bld_mult / bld::bld_mult_cuda
+ * (src/bld/bld_mult/bld_mult.hpp:83-86 and :226-229).
+ * X: x*y; alpha: x; beta: z (Params::alpha / Params::beta,
+ * src/alloc_model/alloc_model_params.hpp:25-84); S_out: x*y*z.
+ * S0 (x*y*z) is the initial tensor; when NULL the solver draws it as
+ * init_S does (bld_mult_funcs.hpp:36-74): S0_ijk = X_ij * Dirichlet(1_z)_k,
+ * from a counter-based stream keyed by `seed`.
+ * Exactly max_iter iterations run (the reference has no convergence test).
+ * Parameter errors (X < 0 or NaN, null pointers) return
+ * BNMF_ERR_INVALID_ARGUMENT, as check_bld_params asserts
+ * (src/bld/util_details.hpp:20-28).
+ * ---------------------------------------------------------------------- */
+/* Plain FP64 form: every step in double, exact digamma unless use_psi_appr. */
+bnmf_status bnmf_bld_mult_f64(const double* X, size_t x, size_t y, size_t z,
+                              const double* alpha, const double* beta,
+                              size_t max_iter, int use_psi_appr, double eps,
+                              const double* S0, uint64_t seed, double* S_out);
+/* FP32-bulk form: S, the update and the axis sums in float with compensated
+ * cross-tile sums; X/alpha/beta as given. */
+bnmf_status bnmf_bld_mult_f32(const float* X, size_t x, size_t y, size_t z,
+                              const double* alpha, const double* beta,
+                              size_t max_iter, int use_psi_appr, double eps,
+                              const float* S0, uint64_t seed, float* S_out);
+
+/* alloc_model::log_marginal_S (src/alloc_model/alloc_model_funcs.hpp:574-586),
+ * evaluated on the GPU in FP64. S: x*y*z of the given dtype. */
+bnmf_status bnmf_log_marginal_S(int dtype, const void* S, size_t x, size_t y, size_t z,
+                                const double* alpha, const double* beta, double a,
+                                double b, double* out);
+
+/* cuda::tensor_sums (src/cuda/tensor_ops.hpp:40-42, cuda_src/tensor_ops.cu:10-55):
+ * S_pjk (y*z) = sum over axis 0, S_ipk (x*z) = axis 1, S_ijp (x*y) = axis 2.
+ * Any output may be NULL. */
+bnmf_status bnmf_tensor_sums(int dtype, const void* S, size_t x, size_t y, size_t z,
+                             void* S_pjk, void* S_ipk, void* S_ijp);
+
+/* ------------------------------------------------------------------------
+ * Batched solver: many independent instances of uneven (x, y, z) advance
+ * together, one iteration per launch pair (prep + fused pass) over a device
+ * work list; the per-iteration chain is replayed from a captured hipGraph.
+ * Inputs are uploaded once; S stays resident until collected.
+ * No counterpart in the reference (it solves one instance per call).
+ * ---------------------------------------------------------------------- */
+typedef struct bnmf_bld_desc {
+    size_t x, y, z;
+    const void* X;        /* host, x*y, of the batch dtype */
+    const double* alpha;  /* host, x */
+    const double* beta;   /* host, z */
+    double a, b;          /* Gamma prior, used by the log-marginal only */
+    const void* S0;       /* host, x*y*z of the batch dtype, or NULL */
+    uint64_t seed;        /* S0 stream when S0 is NULL */
+} bnmf_bld_desc;
+
+typedef struct bnmf_batch bnmf_batch;
+
+bnmf_status bnmf_batch_create(bnmf_batch** out, int dtype, const bnmf_bld_desc* descs,
+                              size_t n, int use_psi_appr, double eps);
+/* Restarts every instance from its S0 and runs max_iter iterations. */
+bnmf_status bnmf_batch_solve(bnmf_batch* b, size_t max_iter);
+/* Same, enqueued on the batch's stream without waiting. */
+bnmf_status bnmf_batch_solve_async(bnmf_batch* b, size_t max_iter);
+bnmf_status bnmf_batch_synchronize(bnmf_batch* b);
+/* Copies instance idx's current S (x*y*z, batch dtype) to host memory. */
+bnmf_status bnmf_batch_get_S(bnmf_batch* b, size_t idx, void* S_out);
+/* FP64 log_marginal_S of every instance's current S; out has n entries. */
+bnmf_status bnmf_batch_log_marginal(bnmf_batch* b, double* out);
+/* Device-side timing: mean milliseconds of one fused pass launch (all
+ * buckets) and of one whole iteration (prep + pass), over `reps`
+ * iterations measured with HIP events on the batch stream. */
+bnmf_status bnmf_batch_time_pass(bnmf_batch* b, int reps, double* pass_ms, double* iter_ms);
+
+typedef struct bnmf_batch_info {
+    size_t n_instances;
+    size_t elements;        /* sum of x*y*z */
+    size_t fibers;          /* sum of x*y */
+    size_t device_bytes;    /* HBM held by the batch */
+    size_t pass_items;      /* tiles per fused pass */
+    size_t prep_items;
+    int n_buckets;          /* distinct z buckets (one pass launch each) */
+} bnmf_batch_info;
+bnmf_status bnmf_batch_get_info(bnmf_batch* b, bnmf_batch_info* info);
+void bnmf_batch_destroy(bnmf_batch* b);
+
+/* ------------------------------------------------------------------------
+ * Input synthesis (host): the allocation model of bnmf_priors / sample_S
+ * (src/alloc_model/alloc_model_funcs.hpp:463-552) from a counter-based
+ * Philox4x32-10 stream, so the output does not depend on the thread count.
+ * W[:,k] ~ Dir(alpha), H[:,j] ~ Dir(beta), L_j ~ Gamma(a, scale b),
+ * S_ijk ~ Poisson(W_ik H_kj L_j).
+ * ---------------------------------------------------------------------- */
+/* X = S_ij+ ~ Poisson(L_j * sum_k W_ik H_kj)  (the sum of the z independent
+ * Poisson draws has exactly this law). X_out: x*y. */
+bnmf_status bnmf_sample_X(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                          const double* beta, double a, double b, double* X_out);
+/* The full tensor S (x*y*z) as sample_S draws it. */
+bnmf_status bnmf_sample_S(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                          const double* beta, double a, double b, double* S_out);
+/* init_S (bld_mult_funcs.hpp:36-74): S0_ijk = X_ij * Dirichlet(1_z)_k. */
+bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_t y, size_t z,
+                        void* S0_out);
+
+/* Evaluates the device digamma forms on the GPU (kind 0: FP32 bulk psi,
+ * 1: exact FP64, 2: util::psi_appr in FP64, src/util/util.hpp:440-514).
+ * x and out are host arrays of n doubles. */
+bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BNMF_HIP_H */

@@ -1,0 +1,246 @@
+"""GPU parity of the HIP solver against the FP64 oracle and golden vectors.
+
+Tolerances (written per test):
+  * plain FP64 form vs the oracle: 1e-10 relative (normwise, as Eigen's
+    isApprox) on S and on log_marginal_S;
+  * FP32-bulk form vs the FP64 oracle: 1e-4 relative normwise on S;
+  * device digamma forms vs mpmath: see each test;
+  * batched vs single-instance runs: bitwise equal.
+Inputs are seeded allocation-model samples (bnmf_hip.sample_X) with seeded
+init_S tensors, passed identically to the GPU and to the oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
+
+
+def _elem_rel(a, b, floor=1e-6):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    den = np.maximum(np.abs(b), floor * max(np.abs(b).max(), 1e-300))
+    return float((np.abs(a - b) / den).max())
+
+
+def _problem(hip, x, y, z, seed, a=40.0, dtype=np.float64):
+    p = hip.Params(a, 1.0, [1.0] * x, [1.0] * z)
+    X = hip.sample_X(seed, (x, y, z), p)
+    S0 = hip.init_S(seed + 1, X, z)
+    return p, X.astype(dtype), S0.astype(dtype)
+
+
+# ------------------------------------------------------------------ digamma
+def test_device_psi_exact(hip, golden):
+    xs = np.array(golden["psi"]["x"])
+    ref = np.array(golden["psi"]["psi"])
+    got = hip.psi_eval(1, xs)
+    assert np.all(np.abs(got - ref) <= 3e-14 * np.maximum(1, np.abs(ref)))
+    neg = hip.psi_eval(1, golden["psi_neg"]["x"])
+    assert np.allclose(neg, golden["psi_neg"]["psi"], rtol=1e-12, atol=1e-12)
+
+
+def test_device_psi_appr(hip, golden):
+    xs = np.array(golden["psi_appr"]["x"])
+    ref = np.array(golden["psi_appr"]["psi_appr"])
+    got = hip.psi_eval(2, xs)
+    assert np.all(np.abs(got - ref) <= 4e-15 * np.maximum(1, np.abs(ref)))
+
+
+def test_device_psi_fp32(hip, oracle, golden):
+    xs = np.array(golden["psi"]["x"])
+    xs = xs[(xs > 1e-3) & (xs < 1e7)]
+    # reference at the float32-rounded argument (the oracle's psi is pinned
+    # to mpmath to 4e-15)
+    xr = xs.astype(np.float32).astype(np.float64)
+    ref = oracle.psi_vec(xr)
+    got = hip.psi_eval(0, xs)
+    assert np.all(np.abs(got - ref) <= 6e-7 * np.maximum(1, np.abs(ref)))
+    hot = xr[xr >= 1]
+    got_hot = hip.psi_eval(3, hot)
+    assert np.all(np.abs(got_hot - oracle.psi_vec(hot)) <= 6e-7 * np.maximum(1, np.abs(oracle.psi_vec(hot))))
+
+
+# ------------------------------------------------------------------ FP64
+@pytest.mark.parametrize("mode", ["exact", "appr"])
+@pytest.mark.parametrize("case", [0, 1])
+def test_fp64_tiny_iterations_match_mpmath(hip, golden, mode, case):
+    c = golden["iterations"][case]
+    X, S0 = np.array(c["X"]), np.array(c["S0"])
+    p = hip.Params(c["a"], c["b"], c["alpha"], c["beta"])
+    z = len(c["beta"])
+    S1 = hip.bld_mult(X, z, p, 1, mode == "appr", S0=S0)
+    assert _rel(S1, c[mode]["S1"]) < 1e-13
+    S5 = hip.bld_mult(X, z, p, 5, mode == "appr", S0=S0)
+    assert _rel(S5, c[mode]["S5"]) < 1e-12
+
+
+SHAPES_F64 = [
+    (100, 100, 17, 1000),  # reference CPU config, full length
+    (356, 478, 17, 20),    # test_bld_mult_cuda_funcs.cpp:61
+    (1024, 456, 17, 10),   # test/cuda/test_tensor_ops.cpp:17
+    (37, 65, 5, 50),
+    (65, 129, 33, 20),
+    (200, 70, 64, 15),
+    (13, 200, 8, 30),
+    (1, 1, 1, 3),
+    (3, 1, 2, 7),
+]
+
+
+@pytest.mark.parametrize("x,y,z,iters", SHAPES_F64)
+def test_fp64_matches_oracle(hip, oracle, x, y, z, iters):
+    p, X, S0 = _problem(hip, x, y, z, seed=x * 1000 + y + z)
+    S_gpu = hip.bld_mult(X, z, p, iters, False, S0=S0)
+    S_cpu = oracle.bld_mult(X, z, p.alpha, p.beta, S0, max_iter=iters)
+    assert _rel(S_gpu, S_cpu) <= 1e-10
+    lm_gpu = hip.log_marginal_S(S_gpu, p)
+    lm_cpu = oracle.log_marginal_S(S_cpu, p.alpha, p.beta, p.a, p.b)
+    assert abs(lm_gpu - lm_cpu) <= 1e-10 * abs(lm_cpu)
+
+
+def test_fp64_psi_appr_matches_oracle(hip, oracle):
+    x, y, z, iters = 100, 100, 17, 300
+    p, X, S0 = _problem(hip, x, y, z, seed=77)
+    S_gpu = hip.bld_mult(X, z, p, iters, True, S0=S0)
+    S_cpu = oracle.bld_mult(X, z, p.alpha, p.beta, S0, max_iter=iters, use_psi_appr=True)
+    assert _rel(S_gpu, S_cpu) <= 1e-10
+
+
+def test_fp64_nonuniform_params(hip, oracle):
+    # alpha/beta differ per index, so a transposed alpha (x) / beta (z)
+    # lookup cannot pass
+    rng = np.random.default_rng(8)
+    x, y, z = 70, 90, 6
+    p = hip.Params(10.0, 2.0, list(rng.uniform(0.05, 3, x)), list(rng.uniform(0.5, 4, z)))
+    X = hip.sample_X(5, (x, y, z), p)
+    S0 = hip.init_S(6, X, z)
+    S_gpu = hip.bld_mult(X, z, p, 40, False, S0=S0)
+    S_cpu = oracle.bld_mult(X, z, p.alpha, p.beta, S0, max_iter=40)
+    assert _rel(S_gpu, S_cpu) <= 1e-10
+
+
+@pytest.mark.parametrize("appr", [False, True])
+def test_fixture_8x10_gpu(hip, oracle, golden, appr):
+    fx = golden["fixture_8x10"]
+    X, S0 = np.array(fx["X"]), np.array(fx["S0"])
+    p = hip.Params(fx["a"], fx["b"], fx["alpha"], fx["beta"])
+    S = hip.bld_mult(X, fx["z"], p, fx["max_iter"], appr, S0=S0)
+    th = fx["thresholds"]
+    assert hip.log_marginal_S(S, p) >= th["log_marginal_min"]
+    assert oracle.sparseness(S) >= th["sparseness_min"]
+    sij = S.sum(2)
+    assert np.linalg.norm(X - sij) / min(np.linalg.norm(X), np.linalg.norm(sij)) <= (
+        th["isapprox_appr"] if appr else th["isapprox_exact"])
+    S_cpu = oracle.bld_mult(X, fx["z"], fx["alpha"], fx["beta"], S0, max_iter=fx["max_iter"],
+                            use_psi_appr=appr)
+    assert _rel(S, S_cpu) <= 1e-10
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_zero_matrix_gives_zero(hip, dtype):
+    p = hip.Params.default((8, 10, 3))
+    S = hip.bld_mult(np.zeros((8, 10)), 3, p, 50, dtype=dtype)
+    assert np.all(S == 0)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_z1_fixed_point(hip, dtype):
+    rng = np.random.default_rng(1)
+    X = rng.integers(0, 9, (66, 70)).astype(dtype)
+    p = hip.Params.default((66, 70, 1))
+    S = hip.bld_mult(X, 1, p, 30, dtype=dtype, S0=X[:, :, None])
+    tol = 1e-13 if dtype == np.float64 else 1e-6
+    assert np.allclose(S[:, :, 0], X, rtol=tol, atol=0)
+
+
+def test_zero_fibers_stay_zero_and_finite(hip):
+    p, X, S0 = _problem(hip, 150, 130, 12, seed=3)
+    X[5, :] = 0
+    X[:, 7] = 0
+    S0 = hip.init_S(4, X, 12)
+    for dt in (np.float64, np.float32):
+        S = hip.bld_mult(X.astype(dt), 12, p, 25, dtype=dt, S0=S0.astype(dt))
+        assert np.all(S[X == 0] == 0)
+        assert np.all(np.isfinite(S))
+
+
+# ------------------------------------------------------------------ FP32
+@pytest.mark.parametrize("x,y,z,iters", [(256, 256, 32, 100), (300, 200, 17, 100),
+                                         (128, 640, 64, 60), (90, 70, 5, 200)])
+def test_fp32_bulk_within_1e4_of_fp64_oracle(hip, oracle, x, y, z, iters):
+    p, X, S0 = _problem(hip, x, y, z, seed=11 * x + z)
+    S32 = hip.bld_mult(X.astype(np.float32), z, p, iters, True, S0=S0.astype(np.float32),
+                       dtype=np.float32)
+    S64 = oracle.bld_mult(X, z, p.alpha, p.beta, S0.astype(np.float32).astype(np.float64),
+                          max_iter=iters, use_psi_appr=True)
+    assert _rel(S32, S64) <= 1e-4
+    lm32 = hip.log_marginal_S(S32, p)
+    lm64 = oracle.log_marginal_S(S64, p.alpha, p.beta, p.a, p.b)
+    assert abs(lm32 - lm64) <= 1e-4 * abs(lm64)
+
+
+# ------------------------------------------------------------ scoring / sums
+@pytest.mark.parametrize("shape", [(64, 64, 8), (70, 130, 17), (5, 300, 64), (1, 1, 1)])
+def test_log_marginal_matches_oracle(hip, oracle, shape):
+    x, y, z = shape
+    rng = np.random.default_rng(x + y + z)
+    S = rng.gamma(0.5, 2.0, shape)
+    p = hip.Params(7.0, 2.0, list(rng.uniform(0.1, 3, x)), list(rng.uniform(0.1, 3, z)))
+    ref = oracle.log_marginal_S(S, p.alpha, p.beta, p.a, p.b)
+    assert hip.log_marginal_S(S, p) == pytest.approx(ref, rel=1e-12)
+    S32 = S.astype(np.float32)
+    ref32 = oracle.log_marginal_S(S32.astype(np.float64), p.alpha, p.beta, p.a, p.b)
+    assert hip.log_marginal_S(S32, p) == pytest.approx(ref32, rel=1e-12)
+
+
+def test_tensor_sums_match_oracle(hip, oracle):
+    # test/cuda/test_tensor_ops.cpp:13-69 shape
+    rng = np.random.default_rng(0)
+    S = rng.uniform(-1, 1, (1024, 456, 17))
+    pjk, ipk, ijp = hip.tensor_sums(S)
+    rp, ri, rij = oracle.tensor_sums(S)
+    assert _rel(pjk, rp) < 1e-13 and _rel(ipk, ri) < 1e-13 and _rel(ijp, rij) < 1e-14
+    p32, i32, ij32 = hip.tensor_sums(S.astype(np.float32))
+    assert _rel(p32, rp) < 1e-6 and _rel(i32, ri) < 1e-6 and _rel(ij32, rij) < 1e-6
+
+
+# ------------------------------------------------------- batch / determinism
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_alone_vs_mix_bitwise(hip, dtype):
+    shapes = [(40, 33, 4), (130, 70, 17), (64, 64, 32), (7, 300, 9), (257, 65, 64)]
+    insts = []
+    for n, (x, y, z) in enumerate(shapes):
+        p, X, S0 = _problem(hip, x, y, z, seed=100 + n)
+        insts.append(hip.Instance(X=X.astype(dtype), z=z, params=p, S0=S0.astype(dtype)))
+    iters = 23
+    appr = dtype == np.float32
+    with hip.Batch(insts, dtype=dtype, use_psi_appr=appr) as b:
+        b.solve(iters)
+        mixed = [b.S(n) for n in range(len(insts))]
+        lms = b.log_marginal()
+    for n, inst in enumerate(insts):
+        alone = hip.bld_mult(inst.X, inst.z, inst.params, iters, appr, S0=inst.S0, dtype=dtype)
+        assert np.array_equal(alone, mixed[n]), shapes[n]
+        assert lms[n] == pytest.approx(hip.log_marginal_S(alone, inst.params), rel=1e-14)
+
+
+def test_repeat_solves_are_bitwise_identical(hip):
+    p, X, S0 = _problem(hip, 300, 200, 20, seed=9)
+    insts = [hip.Instance(X=X.astype(np.float32), z=20, params=p, S0=S0.astype(np.float32))]
+    with hip.Batch(insts, dtype=np.float32) as b:
+        b.solve(40)
+        first = b.S(0)
+        b.solve(40)
+        assert np.array_equal(first, b.S(0))
+
+
+def test_seeded_init_matches_explicit_S0(hip):
+    p, X, _ = _problem(hip, 50, 60, 7, seed=21)
+    S_seed = hip.bld_mult(X, 7, p, 12, seed=1234)
+    S_expl = hip.bld_mult(X, 7, p, 12, S0=hip.init_S(1234, X, 7))
+    assert np.array_equal(S_seed, S_expl)
