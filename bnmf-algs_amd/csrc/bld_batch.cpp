@@ -76,8 +76,8 @@ InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z) {
     L.x = x;
     L.y = y;
     L.z = z;
-    L.zp = z | 1u;
     L.zb = z_bucket(z);
+    read_layout(dtype, z, &L.rw, &L.hs, &L.hm);
     L.ti_rows = choose_ti_rows(dtype, x, y, z);
     L.n_ti = ceil_div(x, L.ti_rows);
     L.n_tj = ceil_div(y, kTileCols);
@@ -90,8 +90,10 @@ InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z) {
         return o;
     };
     const size_t xyz = (size_t)x * y * z;
-    L.off_S = take(xyz * es);
-    L.off_S0 = take(xyz * es);
+    // the pass reads whole 16-byte chunks around each row segment: keep
+    // 64 bytes of slack after the tensors it reads
+    L.off_S = take(xyz * es + 64);
+    L.off_S0 = take(xyz * es + 64);
     L.off_X = take((size_t)x * y * es);
     L.off_alpha = take((size_t)x * 8);
     L.off_beta = take((size_t)z * 8);
@@ -106,7 +108,8 @@ InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z) {
     return L;
 }
 
-BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first) {
+BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first,
+                  bool nonneg) {
     char* p = static_cast<char*>(base);
     BldInst d{};
     d.S = p + L.off_S;
@@ -129,7 +132,9 @@ BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double 
     d.x = L.x;
     d.y = L.y;
     d.z = L.z;
-    d.zp = L.zp;
+    d.rw = L.rw;
+    d.hs = L.hs;
+    d.hm = L.hm;
     d.ti_rows = L.ti_rows;
     d.n_ti = L.n_ti;
     d.n_tj = L.n_tj;
@@ -142,6 +147,7 @@ BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double 
     }
     d.zq = zq;
     d.zq_log2 = lg;
+    d.nonneg = nonneg ? 1u : 0u;
     return d;
 }
 
@@ -151,6 +157,7 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     ensure_runtime();
     int dev = 0;
     BNMF_HIP(hipGetDevice(&dev));
+    dev_ = dev;
     BNMF_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
 
     // layouts and arena
@@ -166,8 +173,11 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     arena_ = DeviceBuffer(std::max<size_t>(total, kAlign));
     device_bytes_ = arena_.bytes;
 
-    // uploads (X, alpha, beta, S0)
+    // uploads (X, alpha, beta, S0). nonneg[n]: S0 >= 0 and every beta_k >= 1,
+    // so grad_plus >= 0 and S stays nonnegative (the kernels then skip the
+    // psi(S+1) guard for S < 0).
     std::vector<char> tmp;
+    std::vector<bool> nonneg(specs.size(), false);
     for (size_t n = 0; n < specs.size(); ++n) {
         const InstSpec& s = specs[n];
         const InstLayout& L = layouts_[n];
@@ -188,6 +198,18 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
                                reinterpret_cast<double*>(tmp.data()));
             S0 = tmp.data();
         }
+        bool nn = true;
+        for (uint32_t k = 0; k < s.z; ++k) nn = nn && s.beta[k] >= 1.0;
+        if (nn && s.S0) {
+            if (dtype == BNMF_F32) {
+                const float* p = static_cast<const float*>(S0);
+                for (size_t t = 0; t < xyz && nn; ++t) nn = p[t] >= 0.0f;
+            } else {
+                const double* p = static_cast<const double*>(S0);
+                for (size_t t = 0; t < xyz && nn; ++t) nn = p[t] >= 0.0;
+            }
+        }
+        nonneg[n] = nn;
         BNMF_HIP(hipMemcpy(b + L.off_S0, S0, xyz * es_, hipMemcpyHostToDevice));
         BNMF_HIP(hipMemcpy(b + L.off_S, S0, xyz * es_, hipMemcpyHostToDevice));
     }
@@ -195,17 +217,22 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     // descriptors and work lists
     std::vector<BldInst> d_first, d_main;
     std::vector<PrepItem> prep;
-    std::vector<std::vector<PassItem>> pass_b(4);
-    std::vector<uint32_t> zp_b(4, 1);
-    auto bidx = [](uint32_t zb) { return zb == 8 ? 0 : zb == 16 ? 1 : zb == 32 ? 2 : 3; };
+    // one pass launch per (z bucket, LDS read width) group
+    std::vector<std::vector<PassItem>> pass_b(12);
+    std::vector<uint32_t> zmax_b(12, 1);
+    auto bidx = [](uint32_t zb, uint32_t rw) {
+        const int zi = zb == 8 ? 0 : zb == 16 ? 1 : zb == 32 ? 2 : 3;
+        const int ri = rw == 1 ? 0 : rw == 2 ? 1 : 2;
+        return zi * 3 + ri;
+    };
     for (size_t n = 0; n < specs.size(); ++n) {
         const InstLayout& L = layouts_[n];
         char* b = arena_.as<char>(base_off[n]);
-        d_first.push_back(make_desc(L, b, eps, specs[n].a, specs[n].b, true));
-        d_main.push_back(make_desc(L, b, eps, specs[n].a, specs[n].b, false));
+        d_first.push_back(make_desc(L, b, eps, specs[n].a, specs[n].b, true, nonneg[n]));
+        d_main.push_back(make_desc(L, b, eps, specs[n].a, specs[n].b, false, nonneg[n]));
         if (L.elements() == 0) continue;
-        const int bi = bidx(L.zb);
-        zp_b[bi] = std::max(zp_b[bi], L.zp);
+        const int bi = bidx(L.zb, L.rw);
+        zmax_b[bi] = std::max(zmax_b[bi], L.z);
         for (uint32_t ti = 0; ti < L.n_ti; ++ti)
             for (uint32_t tj = 0; tj < L.n_tj; ++tj)
                 pass_b[bi].push_back(PassItem{(uint32_t)n, ti, tj, 0});
@@ -223,12 +250,13 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     off_prep_items_ = mtake(prep.size() * sizeof(PrepItem));
     off_prep_count_ = mtake(sizeof(uint32_t));
     n_prep_items_ = (uint32_t)prep.size();
-    const int zbs[4] = {8, 16, 32, 64};
-    for (int bi = 0; bi < 4; ++bi) {
+    const int zbs[4] = {8, 16, 32, 64}, rws[3] = {1, 2, 4};
+    for (int bi = 0; bi < 12; ++bi) {
         if (pass_b[bi].empty()) continue;
         Bucket bk;
-        bk.zb = zbs[bi];
-        bk.zp_max = zp_b[bi];
+        bk.zb = zbs[bi / 3];
+        bk.rw = rws[bi % 3];
+        bk.zmax = zmax_b[bi];
         bk.n_items = (uint32_t)pass_b[bi].size();
         bk.off_items = mtake(pass_b[bi].size() * sizeof(PassItem));
         bk.off_count = mtake(sizeof(uint32_t));
@@ -245,7 +273,7 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         BNMF_HIP(hipMemcpy(m + off_prep_items_, prep.data(), prep.size() * sizeof(PrepItem), hipMemcpyHostToDevice));
     BNMF_HIP(hipMemcpy(m + off_prep_count_, &n_prep_items_, sizeof(uint32_t), hipMemcpyHostToDevice));
     size_t bpos = 0;
-    for (int bi = 0; bi < 4; ++bi) {
+    for (int bi = 0; bi < 12; ++bi) {
         if (pass_b[bi].empty()) continue;
         Bucket& bk = buckets_[bpos++];
         BNMF_HIP(hipMemcpy(m + bk.off_items, pass_b[bi].data(), pass_b[bi].size() * sizeof(PassItem), hipMemcpyHostToDevice));
@@ -266,9 +294,9 @@ void Batch::enqueue_pass(int mode, bool first) {
     const BldInst* d = reinterpret_cast<const BldInst*>(m + (first ? off_desc_first_ : off_desc_));
     for (const Bucket& bk : buckets_) {
         const dim3 grid(bk.n_items);
-        BNMF_HIP(launch_pass(dtype_, psi_, mode, bk.zb, grid, stream_, d,
+        BNMF_HIP(launch_pass(dtype_, psi_, mode, bk.zb, bk.rw, grid, stream_, d,
                              reinterpret_cast<const PassItem*>(m + bk.off_items),
-                             reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zp_max));
+                             reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
     }
 }
 
@@ -422,9 +450,9 @@ void Batch::tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp) {
     BNMF_HIP(hipMemcpy(dd.ptr, &d, sizeof(BldInst), hipMemcpyHostToDevice));
     char* m = meta_.as<char>();
     const Bucket& bk = buckets_.at(0);
-    BNMF_HIP(launch_pass(dtype_, psi_, MODE_SUMS, bk.zb, dim3(bk.n_items), stream_, dd.as<BldInst>(),
-                         reinterpret_cast<const PassItem*>(m + bk.off_items),
-                         reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zp_max));
+    BNMF_HIP(launch_pass(dtype_, psi_, MODE_SUMS, bk.zb, bk.rw, dim3(bk.n_items), stream_,
+                         dd.as<BldInst>(), reinterpret_cast<const PassItem*>(m + bk.off_items),
+                         reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
     BNMF_HIP(launch_prep(dtype_, psi_, true, dim3(n_prep_items_), stream_, dd.as<BldInst>(),
                          reinterpret_cast<const PrepItem*>(m + off_prep_items_),
                          reinterpret_cast<const uint32_t*>(m + off_prep_count_)));

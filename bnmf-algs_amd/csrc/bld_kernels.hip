@@ -89,37 +89,109 @@ __device__ __forceinline__ T recip_fast(T v);
 template <>
 __device__ __forceinline__ float recip_fast<float>(float v) { return __builtin_amdgcn_rcpf(v); }
 
-// Wavefronts per pass block: FP64 fibers of up to 64 elements use two waves
-// so that the block's LDS stays within the 160 KB of a CU.
-template <typename T, int ZB>
-__host__ __device__ constexpr int pass_waves() {
-    return (sizeof(T) == 8 && ZB == 64) ? 2 : 4;
-}
-
-// LDS bytes of the pass kernel for a given odd fiber stride.
-// Byte offset of the fiber buffers: after psib (64*zp), psiA (zp) and the
-// per-wave grad_minus rows (WAVES*zp), rounded up to 16 bytes.
-template <typename T, int WAVES>
-__host__ __device__ constexpr size_t pass_fib_offset(uint32_t zp) {
-    return ((size_t)(64 * zp + zp + WAVES * zp) * sizeof(T) + 15) / 16 * 16;
-}
-
-template <typename T, typename ACC, int WAVES>
-__host__ __device__ constexpr size_t pass_lds_bytes(uint32_t zp) {
-    return pass_fib_offset<T, WAVES>(zp) +
-           (size_t)WAVES * 64 * zp * (sizeof(ACC) > sizeof(T) ? sizeof(ACC) : sizeof(T));
-}
-
 // ---------------------------------------------------------------------------
 // pass kernel
 // ---------------------------------------------------------------------------
-template <typename T, typename ACC, int ZB, int MODE, int PSI, int WAVES>
+// Global-address-space views: pointers read out of the descriptor are generic
+// to the compiler; casting them keeps S traffic on global_load/global_store
+// (flat accesses also count against the LDS wait counter, which would
+// serialise the row prefetch with the LDS work).
+#define BNMF_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const BNMF_GLOBAL T* gconst(const void* p) {
+    return (const BNMF_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ BNMF_GLOBAL T* gmut(void* p) {
+    return (BNMF_GLOBAL T*)p;
+}
+
+// RW-element vector of T (one LDS read of a fiber chunk); RW = 1 is T.
+template <typename T, int RW>
+struct VecR {
+    typedef T type __attribute__((ext_vector_type(RW)));
+};
+template <typename T>
+struct VecR<T, 1> {
+    typedef T type;
+};
+template <typename T, int RW>
+__device__ __forceinline__ T vget(const typename VecR<T, RW>::type& v, int i) {
+    if constexpr (RW == 1) {
+        (void)i;
+        return v;
+    } else {
+        return v[i];
+    }
+}
+template <typename T, int RW>
+__device__ __forceinline__ void vset(typename VecR<T, RW>::type& v, int i, T val) {
+    if constexpr (RW == 1) {
+        (void)i;
+        v = val;
+    } else {
+        v[i] = val;
+    }
+}
+
+__host__ __device__ constexpr size_t round16(size_t b) { return (b + 15) / 16 * 16; }
+
+// LDS layout of a pass block, for an instance of depth z:
+//   [psib: 64*z T][psiA: z T][WAVES x grad_minus row: z T]   (update only)
+//   [WAVES x row buffer: 64*z + 2*VEC slots of max(T, ACC)]
+// A row buffer holds one tile row of S in its natural layout: fiber f,
+// element k at delta + f*z + k, delta = misalignment of the row start.
+template <typename T, typename ACC, int MODE>
+struct PassLds {
+    static constexpr size_t VEC = 16 / sizeof(T);
+    static constexpr size_t SLOT = sizeof(ACC) > sizeof(T) ? sizeof(ACC) : sizeof(T);
+    __host__ __device__ static constexpr size_t psiA_off(uint32_t z) {
+        return MODE == 0 ? round16((size_t)64 * z * sizeof(T)) : 0;
+    }
+    __host__ __device__ static constexpr size_t gm_off(uint32_t z) {
+        return MODE == 0 ? psiA_off(z) + round16((size_t)z * sizeof(T)) : 0;
+    }
+    __host__ __device__ static constexpr size_t gm_stride(uint32_t z) {
+        return MODE == 0 ? round16((size_t)z * sizeof(T)) : 0;
+    }
+    __host__ __device__ static constexpr size_t raw_stride(uint32_t z) {
+        return round16(((size_t)64 * z + 2 * VEC) * SLOT);
+    }
+    __host__ __device__ static constexpr size_t raw_off(uint32_t z, int waves) {
+        return gm_off(z) + (size_t)waves * gm_stride(z);
+    }
+    __host__ __device__ static constexpr size_t bytes(uint32_t z, int waves) {
+        return raw_off(z, waves) + (size_t)waves * raw_stride(z);
+    }
+};
+
+constexpr size_t kLdsBytes = 160 * 1024;
+
+// Wavefronts per pass block. Eight waves (two per SIMD) keep the FP32 fiber
+// state of z <= 32 in 256 VGPRs; deeper fibers and FP64 use four waves so a
+// wave may hold up to 512 registers without spilling. Never more than the
+// LDS of a CU holds.
+template <typename T, typename ACC, int ZB, int MODE>
+__host__ __device__ constexpr int pass_waves() {
+    using Lds = PassLds<T, ACC, MODE>;
+    int w = (sizeof(T) == 4 && ZB <= 32) ? 8 : 4;
+    while (w > 1 && Lds::bytes(ZB, w) > kLdsBytes) --w;
+    return w;
+}
+
+template <typename T, typename ACC, int ZB, int RW, int MODE, int PSI, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
     bld_pass_kernel(const BldInst* __restrict__ insts, const PassItem* __restrict__ items,
                     const uint32_t* __restrict__ n_items_p) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
+    using Lds = PassLds<T, ACC, MODE>;
     constexpr uint32_t NT = 64 * WAVES;
-    constexpr size_t SLOT = sizeof(ACC) > sizeof(T) ? sizeof(ACC) : sizeof(T);
+    constexpr uint32_t VEC = 16 / sizeof(T);
+    constexpr int CPL = (64 * ZB + 2 * VEC + 64 * VEC - 1) / (64 * VEC);  // 16-byte chunks per lane
+    constexpr int NCH = ZB / RW;                                        // fiber chunks
+    typedef T V16 __attribute__((ext_vector_type(VEC)));
+    typedef typename VecR<T, RW>::type VR;
+
     const uint32_t n_items = *n_items_p;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -128,34 +200,33 @@ __global__ void __launch_bounds__(64 * WAVES)
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const PassItem item = items[it];
         const BldInst* __restrict__ I = insts + item.inst;
-        const uint32_t x = I->x, y = I->y, z = I->z, zp = I->zp;
-        const float inv_z = I->inv_z;
+        const uint32_t x = I->x, y = I->y, z = I->z;
         const uint32_t i0 = item.ti * I->ti_rows;
         const uint32_t i1 = min(x, i0 + I->ti_rows);
         const uint32_t j0 = item.tj * kTileCols;
         const uint32_t nj = min(kTileCols, y - j0);
         const uint32_t n_el = nj * z;
+        const uint32_t nchunk = z / RW;                                 // fiber chunks in use
+        const uint32_t hsw = RW > 1 ? ((lane >> I->hs) & I->hm) : 0u;  // visit order
 
         T* const psib_l = reinterpret_cast<T*>(lds_raw);
-        T* const psiA_l = psib_l + 64 * zp;
-        T* const gm_l = psiA_l + zp + w * zp;
-        unsigned char* const fib_bytes = lds_raw + pass_fib_offset<T, WAVES>(zp);
-        T* const fib = reinterpret_cast<T*>(fib_bytes + (size_t)w * 64 * zp * SLOT);
+        T* const psiA_l = reinterpret_cast<T*>(lds_raw + Lds::psiA_off(z));
+        T* const gm_l = reinterpret_cast<T*>(lds_raw + Lds::gm_off(z) + w * Lds::gm_stride(z));
+        unsigned char* const raw_base = lds_raw + Lds::raw_off(z, WAVES);
+        T* const raw = reinterpret_cast<T*>(raw_base + w * Lds::raw_stride(z));
 
-        const T* __restrict__ Ssrc = static_cast<const T*>(I->S_src);
-        T* __restrict__ Sdst = static_cast<T*>(I->S);
+        const BNMF_GLOBAL T* __restrict__ Ssrc = gconst<T>(I->S_src);
+        BNMF_GLOBAL T* __restrict__ Sdst = gmut<T>(I->S);
         const double eps = I->eps;
+        const bool fast_psi = I->nonneg != 0u;
 
         if constexpr (MODE == PASS_UPDATE) {
-            const T* __restrict__ psib_g = static_cast<const T*>(I->psib) + (size_t)j0 * z;
-            for (uint32_t e = tid; e < n_el; e += NT) {
-                const uint32_t f = fiber_of(e, inv_z);
-                psib_l[f * zp + (e - f * z)] = psib_g[e];
-            }
+            const BNMF_GLOBAL T* __restrict__ psib_g = gconst<T>(I->psib) + (size_t)j0 * z;
+            for (uint32_t e = tid; e < n_el; e += NT) psib_l[e] = psib_g[e];
             if (tid < z) {
                 // A_k = sum_i alpha_eph(i,k), the argument of psi in
                 // update_grad_minus (bld_mult_funcs.hpp:213-220)
-                const T* __restrict__ pA = static_cast<const T*>(I->partA);
+                const BNMF_GLOBAL T* __restrict__ pA = gconst<T>(I->partA);
                 SumAcc<T> A;
                 for (uint32_t c = 0; c < I->nA; ++c) A.add(pA[(size_t)c * z + tid]);
                 psiA_l[tid] = psi_sel<PSI, T>(A.get());
@@ -172,72 +243,97 @@ __global__ void __launch_bounds__(64 * WAVES)
         const uint32_t part = lane >> I->zq_log2;
         const uint32_t L = 64u >> I->zq_log2;
 
-        T pf[ZB];
+        // prefetch registers: one tile row as 16-byte chunks from the aligned
+        // start (delta elements before the row's first element)
+        V16 pf[CPL];
         uint32_t i = i0 + w;
-        if (i < i1) {
-            const T* __restrict__ src = Ssrc + ((size_t)i * y + j0) * z;
+        uint32_t dl = 0, nch = 0;
+        auto issue_row = [&](uint32_t row, uint32_t& d_out, uint32_t& n_out) {
+            const size_t s0 = ((size_t)row * y + j0) * z;
+            const uint32_t d = (uint32_t)(s0 % VEC);
+            const uint32_t n = (d + n_el + VEC - 1) / VEC;
+            const BNMF_GLOBAL V16* src = (const BNMF_GLOBAL V16*)(Ssrc + (s0 - d));
+            const uint32_t nfull = n >> 6;
 #pragma unroll
-            for (int c = 0; c < ZB; ++c) {
-                const uint32_t e = (uint32_t)c * 64u + lane;
-                if ((uint32_t)c * 64u < n_el) pf[c] = e < n_el ? src[e] : T(0);
-            }
-        }
-        for (; i < i1; i += WAVES) {
-            // scatter the current row into the wave's fiber buffer
-#pragma unroll
-            for (int c = 0; c < ZB; ++c) {
-                const uint32_t e = (uint32_t)c * 64u + lane;
-                if ((uint32_t)c * 64u < n_el && e < n_el) {
-                    const uint32_t f = fiber_of(e, inv_z);
-                    fib[f * zp + (e - f * z)] = pf[c];
+            for (int c = 0; c < CPL; ++c) {
+                const uint32_t q = (uint32_t)c * 64u + lane;
+                if ((uint32_t)c < nfull) {
+                    pf[c] = src[q];
+                } else if ((uint32_t)c == nfull && q < n) {
+                    pf[c] = src[q];
                 }
             }
+            d_out = d;
+            n_out = n;
+        };
+        if (i < i1) issue_row(i, dl, nch);
+        for (; i < i1; i += WAVES) {
+            // the current row into the wave's row buffer (natural layout)
+            {
+                const uint32_t nfull = nch >> 6;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const uint32_t q = (uint32_t)c * 64u + lane;
+                    if ((uint32_t)c < nfull || ((uint32_t)c == nfull && q < nch))
+                        reinterpret_cast<V16*>(raw)[q] = pf[c];
+                }
+            }
+            const uint32_t cur_dl = dl, cur_nch = nch;
+            const size_t cur_s0 = ((size_t)i * y + j0) * z;
             // issue the next row's loads
             const uint32_t inext = i + WAVES;
-            if (inext < i1) {
-                const T* __restrict__ src = Ssrc + ((size_t)inext * y + j0) * z;
-#pragma unroll
-                for (int c = 0; c < ZB; ++c) {
-                    const uint32_t e = (uint32_t)c * 64u + lane;
-                    if ((uint32_t)c * 64u < n_el) pf[c] = e < n_el ? src[e] : T(0);
-                }
-            }
+            if (inext < i1) issue_row(inext, dl, nch);
 
             if constexpr (MODE == PASS_UPDATE) {
-                const T* __restrict__ psia_g = static_cast<const T*>(I->psia) + (size_t)i * z;
+                const BNMF_GLOBAL T* __restrict__ psia_g = gconst<T>(I->psia) + (size_t)i * z;
                 if (lane < z) gm_l[lane] = psiA_l[lane] - psia_g[lane];
                 T xv = T(0);
-                if (lane < nj) xv = static_cast<const T*>(I->X)[(size_t)i * y + j0 + lane];
+                if (lane < nj) xv = gconst<T>(I->X)[(size_t)i * y + j0 + lane];
                 wave_sync();
                 if (lane < nj) {
-                    T* const my = fib + lane * zp;
-                    const T* const pb = psib_l + lane * zp;
+                    T* const fb = raw + cur_dl + lane * z;
+                    const T* const pbb = psib_l + lane * z;
                     T gp[ZB];
                     T sumS = T(0), nom = T(0), den = T(0);
                     if constexpr (sizeof(T) == 8) {
-                        // FP64 plain form: the reference's operations in its
-                        // order (X_reciprocal :95, nom :254, denom :284-285,
-                        // update :319-323).
+                        // FP64 plain form: the reference's operations
+                        // (X_reciprocal :95, nom :254, denom :284-285,
+                        // update :319-323); fiber sums run in visit order.
                         const T xr = T(1.0) / (xv + eps);
 #pragma unroll
-                        for (int k = 0; k < ZB; ++k) {
-                            if (k < (int)z) {
-                                const T s = my[k];
-                                const T g = pb[k] - psi_sp1<PSI, T>(s);
-                                gp[k] = g;
-                                sumS += s;
-                                nom += s * xr * gm_l[k];
-                                den += s * xr * g;
+                        for (int m = 0; m < NCH; ++m) {
+                            if ((uint32_t)m < nchunk) {
+                                const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                                const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                                const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                                const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+#pragma unroll
+                                for (int v = 0; v < RW; ++v) {
+                                    const T sk = vget<T, RW>(sv, v);
+                                    const T g = vget<T, RW>(pb, v) - psi_sp1<PSI, T>(sk);
+                                    gp[m * RW + v] = g;
+                                    sumS += sk;
+                                    nom += sk * xr * vget<T, RW>(gv, v);
+                                    den += sk * xr * g;
+                                }
                             }
                         }
                         const T x_over_s = xv / (sumS + eps);
 #pragma unroll
-                        for (int k = 0; k < ZB; ++k) {
-                            if (k < (int)z) {
-                                const T s = my[k];
-                                const T sn = s * ((gp[k] + nom) * x_over_s / (gm_l[k] + den + eps));
-                                my[k] = sn;
-                                acc[k] += ACC(sn);
+                        for (int m = 0; m < NCH; ++m) {
+                            if ((uint32_t)m < nchunk) {
+                                const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                                VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                                const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+#pragma unroll
+                                for (int v = 0; v < RW; ++v) {
+                                    const T sk = vget<T, RW>(sv, v);
+                                    const T sn = sk * ((gp[m * RW + v] + nom) * x_over_s /
+                                                       (vget<T, RW>(gv, v) + den + eps));
+                                    vset<T, RW>(sv, v, sn);
+                                    acc[m * RW + v] += ACC(sn);
+                                }
+                                *reinterpret_cast<VR*>(fb + mm) = sv;
                             }
                         }
                     } else {
@@ -249,29 +345,47 @@ __global__ void __launch_bounds__(64 * WAVES)
                         const bool live = xv > 0.0f;
                         const float xr = live ? 1.0f / (xv + epsf) : 0.0f;
 #pragma unroll
-                        for (int k = 0; k < ZB; ++k) {
-                            if (k < (int)z) {
-                                const float s = my[k];
-                                const float g = pb[k] - psi_sp1<PSI, T>(s);
-                                gp[k] = g;
-                                sumS += s;
-                                const float t = s * xr;
-                                nom = fmaf(t, gm_l[k], nom);
-                                den = fmaf(t, g, den);
+                        for (int m = 0; m < NCH; ++m) {
+                            if ((uint32_t)m < nchunk) {
+                                const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                                const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                                const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                                const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+#pragma unroll
+                                for (int v = 0; v < RW; ++v) {
+                                    const float sk = vget<T, RW>(sv, v);
+                                    // S >= 0 throughout when S0 >= 0 and
+                                    // beta >= 1, so s + 1 >= 1: no guard
+                                    const float ps = fast_psi ? psi_ge1_f32(sk + 1.0f)
+                                                              : psi_sp1<PSI, T>(sk);
+                                    const float g = vget<T, RW>(pb, v) - ps;
+                                    gp[m * RW + v] = g;
+                                    sumS += sk;
+                                    const float t = sk * xr;
+                                    nom = fmaf(t, vget<T, RW>(gv, v), nom);
+                                    den = fmaf(t, g, den);
+                                }
                             }
                         }
                         const bool upd = live && sumS != 0.0f;
                         const float x_over_s = upd ? xv / (sumS + epsf) : 0.0f;
                         const float den_e = den + epsf;
 #pragma unroll
-                        for (int k = 0; k < ZB; ++k) {
-                            if (k < (int)z) {
-                                const float s = my[k];
-                                const float fct = (gp[k] + nom) * x_over_s *
-                                                  recip_fast<float>(gm_l[k] + den_e);
-                                const float sn = upd ? s * fct : 0.0f;
-                                my[k] = sn;
-                                acc[k] += ACC(sn);
+                        for (int m = 0; m < NCH; ++m) {
+                            if ((uint32_t)m < nchunk) {
+                                const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                                VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                                const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+#pragma unroll
+                                for (int v = 0; v < RW; ++v) {
+                                    const float sk = vget<T, RW>(sv, v);
+                                    const float fct = (gp[m * RW + v] + nom) * x_over_s *
+                                                      recip_fast<float>(vget<T, RW>(gv, v) + den_e);
+                                    const float sn = upd ? sk * fct : 0.0f;
+                                    vset<T, RW>(sv, v, sn);
+                                    acc[m * RW + v] += ACC(sn);
+                                }
+                                *reinterpret_cast<VR*>(fb + mm) = sv;
                             }
                         }
                     }
@@ -279,79 +393,112 @@ __global__ void __launch_bounds__(64 * WAVES)
             } else {
                 wave_sync();
                 if (lane < nj) {
-                    const T* const my = fib + lane * zp;
+                    const T* const fb = raw + cur_dl + lane * z;
                     ACC fs = ACC(0);
 #pragma unroll
                     for (int k = 0; k < ZB; ++k) {
                         if (k < (int)z) {
-                            const T s = my[k];
-                            acc[k] += ACC(s);
-                            fs += ACC(s);
-                            if constexpr (MODE == PASS_SCORE) lg_sum += lgamma((double)s + 1.0);
+                            const T sk = fb[k];
+                            acc[k] += ACC(sk);
+                            fs += ACC(sk);
+                            if constexpr (MODE == PASS_SCORE) lg_sum += lgamma((double)sk + 1.0);
                         }
                     }
                     if constexpr (MODE == PASS_SUMS)
-                        if (I->sijp != nullptr) static_cast<ACC*>(I->sijp)[(size_t)i * y + j0 + lane] = fs;
+                        if (I->sijp != nullptr) gmut<ACC>(I->sijp)[(size_t)i * y + j0 + lane] = fs;
                 }
             }
             wave_sync();
 
-            // S_i+k partial over the tile's columns (fixed order)
+            // S_i+k partial over the tile's columns: eight independent
+            // partial sums combined in a fixed tree (deterministic)
             {
-                ACC rs = ACC(0);
-                if (kq < z)
-                    for (uint32_t f = part; f < nj; f += L) rs += ACC(fib[f * zp + kq]);
+                ACC r[8];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) r[v] = ACC(0);
+                if (kq < z) {
+                    const T* const rb = raw + cur_dl + kq;
+                    for (uint32_t f0 = part; f0 < nj; f0 += 8u * L) {
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) {
+                            const uint32_t f = f0 + (uint32_t)v * L;
+                            if (f < nj) r[v] += ACC(rb[f * z]);
+                        }
+                    }
+                }
+                ACC rs = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
                 for (uint32_t off = I->zq; off < 64u; off <<= 1) rs += __shfl_xor(rs, (int)off);
                 if (part == 0 && kq < z)
-                    static_cast<ACC*>(I->part_ipk)[((size_t)item.tj * x + i) * z + kq] = rs;
+                    gmut<ACC>(I->part_ipk)[((size_t)item.tj * x + i) * z + kq] = rs;
             }
 
             if constexpr (MODE == PASS_UPDATE) {
-                T* __restrict__ dst = Sdst + ((size_t)i * y + j0) * z;
+                // 16-byte stores of every chunk that lies inside the row
+                // segment; the (at most two) chunks straddling its ends are
+                // written element by element so neighbours are not touched.
+                BNMF_GLOBAL T* __restrict__ dst = Sdst + (cur_s0 - cur_dl);
+                const uint32_t lo = cur_dl, hi = cur_dl + n_el;
+                const uint32_t nfull = cur_nch >> 6;
 #pragma unroll
-                for (int c = 0; c < ZB; ++c) {
-                    const uint32_t e = (uint32_t)c * 64u + lane;
-                    if ((uint32_t)c * 64u < n_el && e < n_el) {
-                        const uint32_t f = fiber_of(e, inv_z);
-                        dst[e] = fib[f * zp + (e - f * z)];
+                for (int c = 0; c < CPL; ++c) {
+                    const uint32_t q = (uint32_t)c * 64u + lane;
+                    if ((uint32_t)c < nfull || ((uint32_t)c == nfull && q < cur_nch)) {
+                        const uint32_t e0 = q * VEC;
+                        if (e0 >= lo && e0 + VEC <= hi) {
+                            reinterpret_cast<BNMF_GLOBAL V16*>(dst)[q] =
+                                reinterpret_cast<const V16*>(raw)[q];
+                        } else {
+                            for (uint32_t v = 0; v < VEC; ++v)
+                                if (e0 + v >= lo && e0 + v < hi) dst[e0 + v] = raw[e0 + v];
+                        }
                     }
                 }
             }
             wave_sync();
         }
 
-        // S_+jk partial over the tile's rows: combine the waves in order
+        // S_+jk partial over the tile's rows: the waves' column sums meet in
+        // LDS (natural fiber layout) and are added in wave order
         __syncthreads();
-        ACC* const comb_self = reinterpret_cast<ACC*>(fib_bytes + (size_t)w * 64 * zp * SLOT);
+        ACC* const comb_self = reinterpret_cast<ACC*>(raw_base + w * Lds::raw_stride(z));
         if (lane < nj) {
+            if constexpr (MODE == PASS_UPDATE) {
 #pragma unroll
-            for (int k = 0; k < ZB; ++k)
-                if (k < (int)z) comb_self[lane * zp + k] = acc[k];
+                for (int m = 0; m < NCH; ++m) {
+                    if ((uint32_t)m < nchunk) {
+                        const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+#pragma unroll
+                        for (int v = 0; v < RW; ++v) comb_self[lane * z + mm + v] = acc[m * RW + v];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < ZB; ++k)
+                    if (k < (int)z) comb_self[lane * z + k] = acc[k];
+            }
         }
         __syncthreads();
         {
-            ACC* __restrict__ pp = static_cast<ACC*>(I->part_pjk) + ((size_t)item.ti * y + j0) * z;
+            BNMF_GLOBAL ACC* __restrict__ pp = gmut<ACC>(I->part_pjk) + ((size_t)item.ti * y + j0) * z;
             for (uint32_t e = tid; e < n_el; e += NT) {
-                const uint32_t f = fiber_of(e, inv_z);
-                const uint32_t o = f * zp + (e - f * z);
-                ACC s = ACC(0);
+                ACC sacc = ACC(0);
 #pragma unroll
                 for (int w2 = 0; w2 < WAVES; ++w2)
-                    s += reinterpret_cast<const ACC*>(fib_bytes + (size_t)w2 * 64 * zp * SLOT)[o];
-                pp[e] = s;
+                    sacc += reinterpret_cast<const ACC*>(raw_base + w2 * Lds::raw_stride(z))[e];
+                pp[e] = sacc;
             }
         }
         if constexpr (MODE == PASS_SCORE) {
             // block sum of the lgamma(S+1) lane sums, fixed tree order
             __syncthreads();
-            double* red = reinterpret_cast<double*>(fib_bytes);
+            double* red = reinterpret_cast<double*>(raw_base);
             for (uint32_t off = 32; off > 0; off >>= 1) lg_sum += __shfl_xor(lg_sum, (int)off);
             if (lane == 0) red[w] = lg_sum;
             __syncthreads();
             if (tid == 0) {
-                double s = 0.0;
-                for (int w2 = 0; w2 < WAVES; ++w2) s += red[w2];
-                I->score[(size_t)item.ti * I->n_tj + item.tj] = s;
+                double sacc = 0.0;
+                for (int w2 = 0; w2 < WAVES; ++w2) sacc += red[w2];
+                I->score[(size_t)item.ti * I->n_tj + item.tj] = sacc;
             }
         }
         __syncthreads();
@@ -562,61 +709,99 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // host-side launchers (explicit instantiation table)
 // ---------------------------------------------------------------------------
-template <typename T, typename ACC, int ZB, int MODE, int PSI>
+template <typename T, typename ACC, int ZB, int RW, int MODE, int PSI>
 static hipError_t launch_pass_t(dim3 grid, hipStream_t st, const BldInst* insts,
-                                const PassItem* items, const uint32_t* n_items, uint32_t zp) {
-    constexpr int WAVES = pass_waves<T, ZB>();
-    const size_t lds = pass_lds_bytes<T, ACC, WAVES>(zp);
-    auto* fn = bld_pass_kernel<T, ACC, ZB, MODE, PSI, WAVES>;
+                                const PassItem* items, const uint32_t* n_items, uint32_t zmax) {
+    constexpr int WAVES = pass_waves<T, ACC, ZB, MODE>();
+    const size_t lds = PassLds<T, ACC, MODE>::bytes(zmax, WAVES);
+    auto* fn = bld_pass_kernel<T, ACC, ZB, RW, MODE, PSI, WAVES>;
     hipLaunchKernelGGL(fn, grid, dim3(64 * WAVES), lds, st, insts, items, n_items);
     return hipGetLastError();
 }
 
-template <typename T, typename ACC, int MODE, int PSI>
+template <typename T, typename ACC, int RW, int MODE, int PSI>
 static hipError_t launch_pass_zb(int zb, dim3 grid, hipStream_t st, const BldInst* insts,
-                                 const PassItem* items, const uint32_t* n, uint32_t zp) {
+                                 const PassItem* items, const uint32_t* n, uint32_t zmax) {
     switch (zb) {
-        case 8: return launch_pass_t<T, ACC, 8, MODE, PSI>(grid, st, insts, items, n, zp);
-        case 16: return launch_pass_t<T, ACC, 16, MODE, PSI>(grid, st, insts, items, n, zp);
-        case 32: return launch_pass_t<T, ACC, 32, MODE, PSI>(grid, st, insts, items, n, zp);
-        case 64: return launch_pass_t<T, ACC, 64, MODE, PSI>(grid, st, insts, items, n, zp);
+        case 8: return launch_pass_t<T, ACC, 8, RW, MODE, PSI>(grid, st, insts, items, n, zmax);
+        case 16: return launch_pass_t<T, ACC, 16, RW, MODE, PSI>(grid, st, insts, items, n, zmax);
+        case 32: return launch_pass_t<T, ACC, 32, RW, MODE, PSI>(grid, st, insts, items, n, zmax);
+        case 64: return launch_pass_t<T, ACC, 64, RW, MODE, PSI>(grid, st, insts, items, n, zmax);
         default: return hipErrorInvalidValue;
     }
 }
 
-// dtype: 0 = float, 1 = double. zp: the largest LDS fiber stride among the
-// instances in the launch (sizes the dynamic LDS).
-hipError_t launch_pass(int dtype, int psi, int mode, int zb, dim3 grid, hipStream_t st,
+template <typename T, typename ACC, int MODE, int PSI>
+static hipError_t launch_pass_rw(int rw, int zb, dim3 grid, hipStream_t st, const BldInst* insts,
+                                 const PassItem* items, const uint32_t* n, uint32_t zmax) {
+    if constexpr (MODE != PASS_UPDATE) {
+        // sums-only and scoring passes read fibers element by element
+        (void)rw;
+        return launch_pass_zb<T, ACC, 1, MODE, PSI>(zb, grid, st, insts, items, n, zmax);
+    } else {
+        if (rw == 1) return launch_pass_zb<T, ACC, 1, MODE, PSI>(zb, grid, st, insts, items, n, zmax);
+        if (rw == 2) return launch_pass_zb<T, ACC, 2, MODE, PSI>(zb, grid, st, insts, items, n, zmax);
+        if constexpr (sizeof(T) == 4) {
+            if (rw == 4) return launch_pass_zb<T, ACC, 4, MODE, PSI>(zb, grid, st, insts, items, n, zmax);
+        }
+        return hipErrorInvalidValue;
+    }
+}
+
+// dtype: 0 = float, 1 = double. rw: LDS read width of the instances in the
+// launch (update mode). zmax: the largest z among them (sizes the LDS).
+hipError_t launch_pass(int dtype, int psi, int mode, int zb, int rw, dim3 grid, hipStream_t st,
                        const BldInst* insts, const PassItem* items, const uint32_t* n_items,
-                       uint32_t zp) {
+                       uint32_t zmax) {
     if (dtype == 0) {
         switch (mode) {
             case PASS_UPDATE:
-                return launch_pass_zb<float, float, PASS_UPDATE, PSI_F32>(zb, grid, st, insts, items, n_items, zp);
+                return launch_pass_rw<float, float, PASS_UPDATE, PSI_F32>(rw, zb, grid, st, insts, items, n_items, zmax);
             case PASS_SUMS:
-                return launch_pass_zb<float, float, PASS_SUMS, PSI_F32>(zb, grid, st, insts, items, n_items, zp);
+                return launch_pass_rw<float, float, PASS_SUMS, PSI_F32>(rw, zb, grid, st, insts, items, n_items, zmax);
             case PASS_SCORE:
-                return launch_pass_zb<float, double, PASS_SCORE, PSI_F32>(zb, grid, st, insts, items, n_items, zp);
+                return launch_pass_rw<float, double, PASS_SCORE, PSI_F32>(rw, zb, grid, st, insts, items, n_items, zmax);
         }
     } else {
         switch (mode) {
             case PASS_UPDATE:
                 if (psi == PSI_F64_APPR)
-                    return launch_pass_zb<double, double, PASS_UPDATE, PSI_F64_APPR>(zb, grid, st, insts, items, n_items, zp);
-                return launch_pass_zb<double, double, PASS_UPDATE, PSI_F64_EXACT>(zb, grid, st, insts, items, n_items, zp);
+                    return launch_pass_rw<double, double, PASS_UPDATE, PSI_F64_APPR>(rw, zb, grid, st, insts, items, n_items, zmax);
+                return launch_pass_rw<double, double, PASS_UPDATE, PSI_F64_EXACT>(rw, zb, grid, st, insts, items, n_items, zmax);
             case PASS_SUMS:
-                return launch_pass_zb<double, double, PASS_SUMS, PSI_F64_EXACT>(zb, grid, st, insts, items, n_items, zp);
+                return launch_pass_rw<double, double, PASS_SUMS, PSI_F64_EXACT>(rw, zb, grid, st, insts, items, n_items, zmax);
             case PASS_SCORE:
-                return launch_pass_zb<double, double, PASS_SCORE, PSI_F64_EXACT>(zb, grid, st, insts, items, n_items, zp);
+                return launch_pass_rw<double, double, PASS_SCORE, PSI_F64_EXACT>(rw, zb, grid, st, insts, items, n_items, zmax);
         }
     }
     return hipErrorInvalidValue;
 }
 
-size_t pass_lds(int dtype, int mode, uint32_t zp) {
+// LDS read width of an instance of depth z (update pass) and the XOR mask of
+// its chunk visit order. A read of RW elements serves 128 / (RW*sizeof(T))
+// lanes per LDS cycle; fibers of lanes l and l' start on the same bank group
+// when l*z and l'*z agree modulo the group count, so lanes that repeat a
+// group (period lanes_per_cycle / P) get distinct XOR masks below P, where P
+// is the largest power of two dividing the chunk count (so m ^ mask stays a
+// permutation of the chunks).
+void read_layout(int dtype, uint32_t z, uint32_t* rw, uint32_t* hs, uint32_t* hm) {
+    uint32_t r = 1;
     if (dtype == 0)
-        return mode == PASS_SCORE ? pass_lds_bytes<float, double, 4>(zp) : pass_lds_bytes<float, float, 4>(zp);
-    return zp > 33 ? pass_lds_bytes<double, double, 2>(zp) : pass_lds_bytes<double, double, 4>(zp);
+        r = (z % 4 == 0) ? 4 : (z % 2 == 0 ? 2 : 1);
+    else
+        r = (z % 2 == 0) ? 2 : 1;
+    const uint32_t es = dtype == 0 ? 4 : 8;
+    const uint32_t lanes_per_cycle = 128u / (r * es);
+    const uint32_t C = z / r;
+    uint32_t P = 1;
+    while (P < lanes_per_cycle && (C % (P * 2)) == 0) P *= 2;
+    if (r == 1) P = 1;
+    const uint32_t period = lanes_per_cycle / P;
+    uint32_t sh = 0;
+    while ((1u << sh) < period) ++sh;
+    *rw = r;
+    *hs = sh;
+    *hm = P - 1;
 }
 
 hipError_t launch_prep(int dtype, int psi, bool raw, dim3 grid, hipStream_t st,
@@ -664,31 +849,35 @@ hipError_t launch_psi_eval(int kind, const double* xin, double* out, size_t n, h
 }
 
 // Sets the dynamic-LDS limit of every pass instantiation once per process.
-template <typename T, typename ACC, int MODE, int PSI>
+template <typename T, typename ACC, int ZB, int RW, int MODE, int PSI>
+static hipError_t attr_one() {
+    constexpr int WAVES = pass_waves<T, ACC, ZB, MODE>();
+    return hipFuncSetAttribute(
+        reinterpret_cast<const void*>(bld_pass_kernel<T, ACC, ZB, RW, MODE, PSI, WAVES>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)PassLds<T, ACC, MODE>::bytes(ZB, WAVES));
+}
+
+template <typename T, typename ACC, int RW, int MODE, int PSI>
 static hipError_t attr_zb() {
-    hipError_t e = hipSuccess;
-#define BNMF_ATTR(ZBV)                                                                          \
-    if (e == hipSuccess)                                                                        \
-        e = hipFuncSetAttribute(                                                                \
-            reinterpret_cast<const void*>(bld_pass_kernel<T, ACC, ZBV, MODE, PSI, pass_waves<T, ZBV>()>), \
-            hipFuncAttributeMaxDynamicSharedMemorySize,                                         \
-            (int)pass_lds_bytes<T, ACC, pass_waves<T, ZBV>()>(ZBV + 1));
-    BNMF_ATTR(8)
-    BNMF_ATTR(16)
-    BNMF_ATTR(32)
-    BNMF_ATTR(64)
-#undef BNMF_ATTR
+    hipError_t e = attr_one<T, ACC, 8, RW, MODE, PSI>();
+    if (e == hipSuccess) e = attr_one<T, ACC, 16, RW, MODE, PSI>();
+    if (e == hipSuccess) e = attr_one<T, ACC, 32, RW, MODE, PSI>();
+    if (e == hipSuccess) e = attr_one<T, ACC, 64, RW, MODE, PSI>();
     return e;
 }
 
 hipError_t init_kernel_attributes() {
-    hipError_t e = attr_zb<float, float, PASS_UPDATE, PSI_F32>();
-    if (e == hipSuccess) e = attr_zb<float, float, PASS_SUMS, PSI_F32>();
-    if (e == hipSuccess) e = attr_zb<float, double, PASS_SCORE, PSI_F32>();
-    if (e == hipSuccess) e = attr_zb<double, double, PASS_UPDATE, PSI_F64_EXACT>();
-    if (e == hipSuccess) e = attr_zb<double, double, PASS_UPDATE, PSI_F64_APPR>();
-    if (e == hipSuccess) e = attr_zb<double, double, PASS_SUMS, PSI_F64_EXACT>();
-    if (e == hipSuccess) e = attr_zb<double, double, PASS_SCORE, PSI_F64_EXACT>();
+    hipError_t e = attr_zb<float, float, 1, PASS_UPDATE, PSI_F32>();
+    if (e == hipSuccess) e = attr_zb<float, float, 2, PASS_UPDATE, PSI_F32>();
+    if (e == hipSuccess) e = attr_zb<float, float, 4, PASS_UPDATE, PSI_F32>();
+    if (e == hipSuccess) e = attr_zb<float, float, 1, PASS_SUMS, PSI_F32>();
+    if (e == hipSuccess) e = attr_zb<float, double, 1, PASS_SCORE, PSI_F32>();
+    if (e == hipSuccess) e = attr_zb<double, double, 1, PASS_UPDATE, PSI_F64_EXACT>();
+    if (e == hipSuccess) e = attr_zb<double, double, 2, PASS_UPDATE, PSI_F64_EXACT>();
+    if (e == hipSuccess) e = attr_zb<double, double, 1, PASS_UPDATE, PSI_F64_APPR>();
+    if (e == hipSuccess) e = attr_zb<double, double, 2, PASS_UPDATE, PSI_F64_APPR>();
+    if (e == hipSuccess) e = attr_zb<double, double, 1, PASS_SUMS, PSI_F64_EXACT>();
+    if (e == hipSuccess) e = attr_zb<double, double, 1, PASS_SCORE, PSI_F64_EXACT>();
     return e;
 }
 
@@ -703,3 +892,4 @@ hipError_t launch_score(dim3 grid, hipStream_t st, const BldInst* insts, const P
 }
 
 }  // namespace bnmf
+

@@ -44,14 +44,17 @@ struct BldInst {
     double eps;
     double a, b;          // Gamma prior (log-marginal only)
     float inv_z;          // 1.0f / z (fiber index from element index)
-    uint32_t x, y, z, zp; // zp: LDS fiber stride (z rounded up to odd)
+    uint32_t x, y, z;
+    uint32_t rw;          // elements per LDS fiber read: 4 (z%4==0), 2 (z even), 1
+    uint32_t hs, hm;      // lane l visits its fiber's chunks in the order
+                          // m ^ ((l >> hs) & hm): distinct LDS banks per step
     uint32_t ti_rows;     // rows per pass tile
     uint32_t n_ti, n_tj;  // tiles along rows / columns
     uint32_t nA;          // prep row chunks
     uint32_t n_cj;        // prep column chunks
     uint32_t zq;          // next power of two >= z
     uint32_t zq_log2;
-    uint32_t pad0;
+    uint32_t nonneg;      // 1: S0 >= 0 and beta >= 1, so S stays >= 0 (fast psi(S+1))
 };
 
 struct PassItem {

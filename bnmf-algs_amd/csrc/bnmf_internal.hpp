@@ -39,10 +39,10 @@ void clear_last_error();
 // ----------------------------------------------------------------- kernels
 enum : int { MODE_UPDATE = 0, MODE_SUMS = 1, MODE_SCORE = 2 };
 enum : int { PSIM_F32 = 0, PSIM_F64_EXACT = 1, PSIM_F64_APPR = 2 };
-hipError_t launch_pass(int dtype, int psi, int mode, int zb, dim3 grid, hipStream_t st,
+hipError_t launch_pass(int dtype, int psi, int mode, int zb, int rw, dim3 grid, hipStream_t st,
                        const BldInst* insts, const PassItem* items, const uint32_t* n_items,
-                       uint32_t zp);
-size_t pass_lds(int dtype, int mode, uint32_t zp);
+                       uint32_t zmax);
+void read_layout(int dtype, uint32_t z, uint32_t* rw, uint32_t* hs, uint32_t* hm);
 hipError_t launch_prep(int dtype, int psi, bool raw, dim3 grid, hipStream_t st,
                        const BldInst* insts, const PrepItem* items, const uint32_t* n_items);
 hipError_t launch_score(dim3 grid, hipStream_t st, const BldInst* insts, const PrepItem* items,
@@ -98,14 +98,15 @@ struct InstSpec {
 
 // Byte layout of one instance inside an arena.
 struct InstLayout {
-    uint32_t x, y, z, zp, zb, ti_rows, n_ti, n_tj, nA, n_cj;
+    uint32_t x, y, z, zb, rw, hs, hm, ti_rows, n_ti, n_tj, nA, n_cj;
     size_t off_S, off_S0, off_X, off_alpha, off_beta, off_psib, off_psia, off_partA, off_pjk,
         off_ipk, off_score, off_result;
     size_t bytes;  // total, 256-byte aligned
     size_t elements() const { return (size_t)x * y * z; }
 };
 InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z);
-BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first);
+BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first,
+                  bool nonneg = false);
 
 // A fixed set of instances sharing one arena, one stream and one set of
 // captured per-iteration graphs.
@@ -124,8 +125,8 @@ class Batch {
 
    private:
     struct Bucket {
-        int zb;
-        uint32_t zp_max;
+        int zb, rw;
+        uint32_t zmax;
         uint32_t n_items;
         size_t off_items;  // in meta_
         size_t off_count;
@@ -135,6 +136,7 @@ class Batch {
     void capture_graphs();
 
     int dtype_, psi_;
+    int dev_ = 0;
     double eps_;
     size_t es_;
     std::vector<InstLayout> layouts_;

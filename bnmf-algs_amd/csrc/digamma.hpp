@@ -143,7 +143,7 @@ __device__ inline float psi_ge1_f32(float x) {
     return __logf(x) + psi32_q(u);
 }
 #else
-inline float psi_ge1_f32(float x) { return logf(x) + psi32_q(1.0f / x); }
+__host__ __device__ inline float psi_ge1_f32(float x) { return logf(x) + psi32_q(1.0f / x); }
 #endif
 
 // any x > 0 (prep arguments S_i+k + alpha_i, S_+jk + beta_k, their column sums)

@@ -89,6 +89,14 @@ SHAPES_F64 = [
     (13, 200, 8, 30),
     (1, 1, 1, 3),
     (3, 1, 2, 7),
+    (50, 77, 6, 12),       # z = 2 mod 4 with odd y: rows start misaligned
+    (33, 99, 10, 12),
+    (70, 91, 12, 12),      # z = 4 mod 8
+    (40, 71, 24, 10),
+    (30, 65, 48, 8),
+    (20, 67, 16, 10),
+    (9, 130, 2, 20),
+    (17, 66, 4, 20),
 ]
 
 
@@ -171,7 +179,11 @@ def test_zero_fibers_stay_zero_and_finite(hip):
 
 # ------------------------------------------------------------------ FP32
 @pytest.mark.parametrize("x,y,z,iters", [(256, 256, 32, 100), (300, 200, 17, 100),
-                                         (128, 640, 64, 60), (90, 70, 5, 200)])
+                                         (128, 640, 64, 60), (90, 70, 5, 200),
+                                         (77, 93, 6, 80), (61, 131, 12, 80),
+                                         (45, 77, 24, 60), (39, 101, 48, 40),
+                                         (50, 67, 16, 80), (70, 99, 8, 80),
+                                         (33, 65, 2, 80), (40, 129, 62, 30)])
 def test_fp32_bulk_within_1e4_of_fp64_oracle(hip, oracle, x, y, z, iters):
     p, X, S0 = _problem(hip, x, y, z, seed=11 * x + z)
     S32 = hip.bld_mult(X.astype(np.float32), z, p, iters, True, S0=S0.astype(np.float32),
