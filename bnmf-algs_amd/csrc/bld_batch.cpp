@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -61,13 +62,33 @@ uint32_t z_bucket(uint32_t z) {
 
 uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
     (void)dtype;
-    (void)y;
     (void)z;
-    // Row-tile height trades the S_+jk partial traffic (1/TI of S) against
-    // the number of tiles available to fill 256 CUs.
-    if (x >= 1024) return 64;
-    if (x >= 256) return 32;
-    return 16;
+    // Row-tile height depends on the instance shape only (never on the rest
+    // of a batch), so an instance sums in the same order alone or batched.
+    // Tall tiles (up to 128 rows) amortise the tile prologue/epilogue and
+    // cut the S_+jk partial traffic to 1/TI of S; an instance with a million
+    // fibers or more takes shorter tiles if that is what it needs to give
+    // 256 CUs two tiles each when it runs alone.
+    const uint32_t n_tj = (y + kTileCols - 1) / kTileCols;
+    // Measured on MI355X: small instances (batched) prefer 64-row tiles,
+    // instances of a million fibers or more 128-row tiles. BNMF_TI_CAP
+    // overrides the cap for experiments.
+    static const long env_cap = [] {
+        const char* e = std::getenv("BNMF_TI_CAP");
+        return e ? std::strtol(e, nullptr, 10) : 0L;
+    }();
+    const bool large = (uint64_t)x * y >= (1ull << 20);
+    const uint32_t cap = env_cap > 0 ? (uint32_t)std::min<long>(std::max<long>(env_cap, 4), 1024)
+                                     : (large ? 128u : 64u);
+    uint32_t nt = (x + cap - 1) / cap;
+    if (large) {
+        const uint32_t fill = (512 + n_tj - 1) / n_tj;
+        const uint32_t most = (x + 15) / 16;
+        nt = std::max(nt, std::min(fill, most));
+    }
+    uint32_t ti = (x + nt - 1) / nt;
+    ti = (ti + 3) / 4 * 4;
+    return std::max<uint32_t>(ti, 4);
 }
 
 InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z) {
@@ -257,6 +278,14 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         bk.zb = zbs[bi / 3];
         bk.rw = rws[bi % 3];
         bk.zmax = zmax_b[bi];
+        bk.work = 0;
+        for (const PassItem& pi : pass_b[bi]) {
+            const InstLayout& L = layouts_[pi.inst];
+            const uint32_t rows = std::min(L.ti_rows, L.x - pi.ti * L.ti_rows);
+            const uint32_t cols = std::min(kTileCols, L.y - pi.tj * kTileCols);
+            bk.work += (size_t)rows * cols * L.z;
+        }
+        bk.lane = 0;
         bk.n_items = (uint32_t)pass_b[bi].size();
         bk.off_items = mtake(pass_b[bi].size() * sizeof(PassItem));
         bk.off_count = mtake(sizeof(uint32_t));
@@ -280,23 +309,61 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         BNMF_HIP(hipMemcpy(m + bk.off_count, &bk.n_items, sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     BNMF_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    // spread the bucket passes over up to four streams, largest work first
+    // onto the least loaded stream
+    const int ns = (int)std::min<size_t>(4, buckets_.size());
+    if (ns > 1) {
+        std::vector<size_t> order(buckets_.size()), load(ns, 0);
+        for (size_t b = 0; b < order.size(); ++b) order[b] = b;
+        std::sort(order.begin(), order.end(),
+                  [&](size_t a, size_t b) { return buckets_[a].work > buckets_[b].work; });
+        for (size_t b : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            buckets_[b].lane = l;
+            load[l] += buckets_[b].work;
+        }
+        side_.resize(ns);
+        fork_ev_.resize(ns);
+        join_ev_.resize(ns);
+        side_[0] = stream_;
+        for (int l = 1; l < ns; ++l) BNMF_HIP(hipStreamCreateWithFlags(&side_[l], hipStreamNonBlocking));
+        for (int l = 0; l < ns; ++l) {
+            BNMF_HIP(hipEventCreateWithFlags(&fork_ev_[l], hipEventDisableTiming));
+            BNMF_HIP(hipEventCreateWithFlags(&join_ev_[l], hipEventDisableTiming));
+        }
+    }
 }
 
 Batch::~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph1_) (void)hipGraphExecDestroy(graph1_);
     if (graphU_) (void)hipGraphExecDestroy(graphU_);
+    for (size_t l = 1; l < side_.size(); ++l) (void)hipStreamDestroy(side_[l]);
+    for (hipEvent_t e : fork_ev_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : join_ev_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Batch::enqueue_pass(int mode, bool first) {
     char* m = meta_.as<char>();
     const BldInst* d = reinterpret_cast<const BldInst*>(m + (first ? off_desc_first_ : off_desc_));
+    const bool fork = side_.size() > 1;
+    if (fork) {
+        BNMF_HIP(hipEventRecord(fork_ev_[0], stream_));
+        for (size_t l = 1; l < side_.size(); ++l) BNMF_HIP(hipStreamWaitEvent(side_[l], fork_ev_[0], 0));
+    }
     for (const Bucket& bk : buckets_) {
         const dim3 grid(bk.n_items);
-        BNMF_HIP(launch_pass(dtype_, psi_, mode, bk.zb, bk.rw, grid, stream_, d,
+        hipStream_t st = fork ? side_[bk.lane] : stream_;
+        BNMF_HIP(launch_pass(dtype_, psi_, mode, bk.zb, bk.rw, grid, st, d,
                              reinterpret_cast<const PassItem*>(m + bk.off_items),
                              reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
+    }
+    if (fork) {
+        for (size_t l = 1; l < side_.size(); ++l) {
+            BNMF_HIP(hipEventRecord(join_ev_[l], side_[l]));
+            BNMF_HIP(hipStreamWaitEvent(stream_, join_ev_[l], 0));
+        }
     }
 }
 

@@ -167,14 +167,13 @@ struct PassLds {
 
 constexpr size_t kLdsBytes = 160 * 1024;
 
-// Wavefronts per pass block. Eight waves (two per SIMD) keep the FP32 fiber
-// state of z <= 32 in 256 VGPRs; deeper fibers and FP64 use four waves so a
-// wave may hold up to 512 registers without spilling. Never more than the
-// LDS of a CU holds.
+// Wavefronts per pass block: four (one per SIMD), so that two blocks can
+// share a CU and hide each other's tile prologue and epilogue; fewer when
+// the LDS of a CU cannot hold four row buffers.
 template <typename T, typename ACC, int ZB, int MODE>
 __host__ __device__ constexpr int pass_waves() {
     using Lds = PassLds<T, ACC, MODE>;
-    int w = (sizeof(T) == 4 && ZB <= 32) ? 8 : 4;
+    int w = 4;
     while (w > 1 && Lds::bytes(ZB, w) > kLdsBytes) --w;
     return w;
 }
@@ -344,28 +343,36 @@ __global__ void __launch_bounds__(64 * WAVES)
                         const float epsf = static_cast<float>(eps);
                         const bool live = xv > 0.0f;
                         const float xr = live ? 1.0f / (xv + epsf) : 0.0f;
+                        // The psi form is chosen once per fiber (a uniform
+                        // branch); inside, the chunk loop is straight-line
+                        // code the compiler can schedule across elements.
+                        auto phase_a = [&](auto psi_of) {
 #pragma unroll
-                        for (int m = 0; m < NCH; ++m) {
-                            if ((uint32_t)m < nchunk) {
-                                const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
-                                const VR sv = *reinterpret_cast<const VR*>(fb + mm);
-                                const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
-                                const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+                            for (int m = 0; m < NCH; ++m) {
+                                if ((uint32_t)m < nchunk) {
+                                    const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                                    const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                                    const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                                    const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
 #pragma unroll
-                                for (int v = 0; v < RW; ++v) {
-                                    const float sk = vget<T, RW>(sv, v);
-                                    // S >= 0 throughout when S0 >= 0 and
-                                    // beta >= 1, so s + 1 >= 1: no guard
-                                    const float ps = fast_psi ? psi_ge1_f32(sk + 1.0f)
-                                                              : psi_sp1<PSI, T>(sk);
-                                    const float g = vget<T, RW>(pb, v) - ps;
-                                    gp[m * RW + v] = g;
-                                    sumS += sk;
-                                    const float t = sk * xr;
-                                    nom = fmaf(t, vget<T, RW>(gv, v), nom);
-                                    den = fmaf(t, g, den);
+                                    for (int v = 0; v < RW; ++v) {
+                                        const float sk = vget<T, RW>(sv, v);
+                                        const float g = vget<T, RW>(pb, v) - psi_of(sk);
+                                        gp[m * RW + v] = g;
+                                        sumS += sk;
+                                        const float t = sk * xr;
+                                        nom = fmaf(t, vget<T, RW>(gv, v), nom);
+                                        den = fmaf(t, g, den);
+                                    }
                                 }
                             }
+                        };
+                        if (fast_psi) {
+                            // S >= 0 throughout when S0 >= 0 and beta >= 1,
+                            // so s + 1 >= 1: no guard for negative S
+                            phase_a([](float sk) { return psi_ge1_f32(sk + 1.0f); });
+                        } else {
+                            phase_a([](float sk) { return psi_sp1<PSI_F32, float>(sk); });
                         }
                         const bool upd = live && sumS != 0.0f;
                         const float x_over_s = upd ? xv / (sumS + epsf) : 0.0f;

@@ -126,6 +126,8 @@ class Batch {
    private:
     struct Bucket {
         int zb, rw;
+        int lane;          // side stream the bucket's pass runs on
+        size_t work;       // elements
         uint32_t zmax;
         uint32_t n_items;
         size_t off_items;  // in meta_
@@ -147,6 +149,11 @@ class Batch {
     uint32_t n_prep_items_ = 0;
     std::vector<Bucket> buckets_;
     hipStream_t stream_ = nullptr;
+    // z-bucket passes of one iteration are independent: they run on up to
+    // four streams forked from and joined back to stream_ (graph branches
+    // when captured)
+    std::vector<hipStream_t> side_;
+    std::vector<hipEvent_t> fork_ev_, join_ev_;
     hipGraphExec_t graph1_ = nullptr, graphU_ = nullptr;
     int unroll_ = 8;
     int cus_ = 256;
