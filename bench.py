@@ -22,7 +22,13 @@ Also measured (same JSON line):
     solver, which cannot be built here) on the reference's CPU config
     100x100x17, 1000 iterations, exact digamma.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--serving]
+  * serving: Poisson arrivals of config-4 requests (x, y log-uniform in
+    [64, 2048], z log-uniform in [8, 64], FP32, 200 iterations) into the
+    continuous-batching server at half its measured saturation rate, with
+    p50/p99 latency and solves/s beside the one-at-a-time FP64 loop
+    (--requests, default 1000; the full configuration is 10000).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--requests N]
 """
 from __future__ import annotations
 
@@ -87,6 +93,106 @@ def rel_err(a, b):
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
 
 
+def make_pool(hip, n, seed=0xB1D4):
+    """Config-4 request inputs: x, y log-uniform in [64, 2048], z log-uniform
+    in [8, 64], allocation-model X with a = 40, b = 1, alpha = beta = 1."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for t in range(n):
+        x = int(round(np.exp(rng.uniform(np.log(64), np.log(2048)))))
+        y = int(round(np.exp(rng.uniform(np.log(64), np.log(2048)))))
+        z = int(round(np.exp(rng.uniform(np.log(8), np.log(64)))))
+        p = hip.Params(40.0, 1.0, [1.0] * x, [1.0] * z)
+        X = hip.sample_X(seed + 31 * t, (x, y, z), p).astype(np.float32)
+        pool.append(hip.Instance(X=X, z=z, params=p, seed=seed + 31 * t + 1))
+    return pool
+
+
+def serving(hip, n_requests, rate_frac, log, iters=200, pool_size=48):
+    """Poisson arrivals into the continuous-batching server (FP32, psi_appr,
+    200 iterations each, S0 drawn on the GPU). Requests cycle through a
+    seeded pool of distinct inputs. Saturation is measured first with a
+    burst; the stream then arrives at rate_frac of it. Latency = completion
+    minus submission (S resident in HBM), steady clock."""
+    import threading
+    log(f"[bench] serving: building a pool of {pool_size} request inputs")
+    pool = make_pool(hip, pool_size)
+    elems = [i.X.shape[0] * i.X.shape[1] * i.z for i in pool]
+    budget = 96 << 30
+    # saturation: a burst of pool_size requests, all queued at once
+    with hip.Server(budget, dtype=np.float32, use_psi_appr=True) as srv:
+        t0 = time.perf_counter()
+        ts = [srv.submit(i, iters) for i in pool]
+        for t in ts:
+            srv.wait(t)
+            srv.collect(t, want_S=False)
+        t_burst = time.perf_counter() - t0
+    sat = pool_size / t_burst
+    rate = rate_frac * sat
+    log(f"[bench] serving: saturation {sat:.1f} req/s ({sum(elems)*iters/t_burst:.3e} upd/s); "
+        f"stream at {rate:.1f} req/s, {n_requests} requests")
+    rng = np.random.default_rng(7)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    picks = rng.integers(0, pool_size, n_requests)
+    lat, done_elems = [], 0
+    with hip.Server(budget, dtype=np.float32, use_psi_appr=True) as srv:
+        tickets = [None] * n_requests
+        ready = threading.Semaphore(0)
+
+        def collector():
+            for k in range(n_requests):
+                ready.acquire()
+                srv.wait(tickets[k])
+                _, _, st = srv.collect(tickets[k], want_S=False)
+                lat.append(st["done_s"] - st["submit_s"])
+                if (k + 1) % max(1, n_requests // 10) == 0:
+                    log(f"[bench] serving: {k + 1}/{n_requests} done")
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t_start = time.perf_counter()
+        next_t = t_start
+        for k in range(n_requests):
+            next_t += gaps[k]
+            d = next_t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            inst = pool[picks[k]]
+            tickets[k] = srv.submit(hip.Instance(X=inst.X, z=inst.z, params=inst.params,
+                                                 seed=inst.seed + k), iters)
+            done_elems += elems[picks[k]]
+            ready.release()
+        th.join()
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+    lat = np.array(lat)
+    # the plain one-at-a-time FP64 GPU loop on the first requests of the pool
+    n64 = 12
+    t0 = time.perf_counter()
+    lat64 = []
+    for inst in pool[:n64]:
+        t1 = time.perf_counter()
+        hip.bld_mult(inst.X.astype(np.float64), inst.z, inst.params, iters, False,
+                     seed=inst.seed, dtype=np.float64)
+        lat64.append(time.perf_counter() - t1)
+    t64 = time.perf_counter() - t0
+    res = {
+        "requests": n_requests, "arrival": "poisson", "rate_req_per_s": rate,
+        "rate_frac_of_saturation": rate_frac, "saturation_req_per_s": sat,
+        "solves_per_s": n_requests / wall, "updates_per_s": done_elems * iters / wall,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3),
+        "peak_active": st["peak_active"], "peak_hbm_bytes": st["peak_bytes"],
+        "hbm_budget_bytes": budget, "iterations": iters, "dtype": "f32", "pool": pool_size,
+        "fp64_one_at_a_time": {"solves_per_s": n64 / t64,
+                               "p50_ms": float(np.percentile(lat64, 50) * 1e3),
+                               "requests": n64, "digamma": "exact"},
+    }
+    log(f"[bench] serving: {res['solves_per_s']:.1f} solves/s, p50 {res['p50_ms']:.1f} ms, "
+        f"p99 {res['p99_ms']:.1f} ms; FP64 one-at-a-time {n64/t64:.2f} solves/s")
+    return res
+
+
 def cpu_baseline(hip):
     from oracle import oracle
     inst = single(hip, 100, 100, 17, 40.0, 0xB1D0)
@@ -111,8 +217,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--serving", action="store_true", help="also run the serving workload")
-    ap.add_argument("--requests", type=int, default=10000)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serving", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--requests", type=int, default=1000,
+                    help="serving requests (the full configuration is 10000)")
+    ap.add_argument("--rate-frac", type=float, default=0.5,
+                    help="serving arrival rate as a fraction of measured saturation")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-loop", action="store_true")
     ap.add_argument("--only-headline", action="store_true")
@@ -218,9 +328,8 @@ def main():
         log(f"[bench] config2: {t2*1e3:.1f} ms, pass {p2*1e3:.1f} us, "
             f"err {out['config2_fp32']['rel_err_S_vs_fp64']:.2e}")
 
-    if args.serving:
-        import serving_bench
-        out["serving"] = serving_bench.run(hip, args.requests, log)
+    if not args.no_serving and (args.serving or not args.only_headline):
+        out["serving"] = serving(hip, args.requests, args.rate_frac, log)
 
     cpu = None
     if not args.no_cpu:

@@ -121,7 +121,7 @@ def lib():
                                              ctypes.c_int, ctypes.c_double]
             L.bnmf_server_create.restype = st
             L.bnmf_server_submit.argtypes = [_VP, ctypes.POINTER(bnmf_bld_desc), _SZ,
-                                             ctypes.POINTER(ctypes.c_uint64)]
+                                             ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
             L.bnmf_server_submit.restype = st
             L.bnmf_server_poll.argtypes = [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
             L.bnmf_server_poll.restype = st
@@ -140,12 +140,18 @@ def lib():
     return _lib
 
 
+class RequestRejected(RuntimeError):
+    """A serving request needs more HBM than the server's whole budget."""
+
+
 def _check(status: int, what: str):
     if status == OK:
         return
     msg = lib().bnmf_last_error().decode()
     if status == ERR_INVALID_ARGUMENT:
         raise ValueError(f"{what}: {msg}")
+    if status == ERR_REJECTED:
+        raise RequestRejected(f"{what}: {msg}")
     if status == ERR_OUT_OF_MEMORY:
         raise MemoryError(f"{what}: {msg}")
     raise RuntimeError(f"{what}: status {status}: {msg}")
@@ -381,6 +387,80 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             lib().bnmf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Server:
+    """Continuous-batching solve server on the GPU (C ABI bnmf_server_*).
+
+    Requests are admitted in arrival order while their HBM footprint fits in
+    `hbm_budget` bytes; active requests advance together and new ones join at
+    the next scheduling step. S stays in HBM until collected."""
+
+    def __init__(self, hbm_budget: int, dtype=np.float32, use_psi_appr: bool = True,
+                 eps: float = 1e-50, max_active: int = 0):
+        self.dt = _np_dtype(dtype)
+        h = _VP()
+        _check(lib().bnmf_server_create(ctypes.byref(h), _code(self.dt), int(hbm_budget),
+                                        int(max_active), int(bool(use_psi_appr)), float(eps)),
+               "server_create")
+        self._h = h
+        self._shapes = {}
+
+    @staticmethod
+    def request_bytes(shape, dtype=np.float32) -> int:
+        x, y, z = shape
+        return int(lib().bnmf_server_request_bytes(_code(_np_dtype(dtype)), x, y, z))
+
+    def submit(self, inst: Instance, max_iter: int, log_marginal: bool = False) -> int:
+        arr, keep = _descs([inst], self.dt)
+        t = ctypes.c_uint64()
+        _check(lib().bnmf_server_submit(self._h, arr, int(max_iter), int(bool(log_marginal)),
+                                        ctypes.byref(t)), "server_submit")
+        del keep
+        x, y = np.shape(inst.X)
+        self._shapes[t.value] = (x, y, inst.z)
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        d = ctypes.c_int()
+        _check(lib().bnmf_server_poll(self._h, int(ticket), ctypes.byref(d)), "server_poll")
+        return bool(d.value)
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> int:
+        return int(lib().bnmf_server_wait(self._h, int(ticket), float(timeout)))
+
+    def collect(self, ticket: int, want_S: bool = True):
+        """Returns (S or None, log_marginal, stats dict); raises on failure."""
+        shape = self._shapes.pop(int(ticket), None)
+        S = np.empty(shape, dtype=self.dt) if (want_S and shape is not None) else None
+        lm = np.zeros(1)
+        st = bnmf_request_stats()
+        _check(lib().bnmf_server_collect(self._h, int(ticket), _ptr(S), _dptr(lm),
+                                         ctypes.byref(st)), "server_collect")
+        stats = {k: getattr(st, k) for k, _ in bnmf_request_stats._fields_}
+        return S, float(lm[0]), stats
+
+    def stats(self) -> dict:
+        st = bnmf_server_stats()
+        _check(lib().bnmf_server_get_stats(self._h, ctypes.byref(st)), "server_get_stats")
+        return {k: getattr(st, k) for k, _ in bnmf_server_stats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bnmf_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

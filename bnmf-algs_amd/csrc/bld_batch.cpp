@@ -91,7 +91,7 @@ uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
     return std::max<uint32_t>(ti, 4);
 }
 
-InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z) {
+InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z, bool with_S0) {
     InstLayout L{};
     const size_t es = dtype == BNMF_F32 ? 4 : 8;
     L.x = x;
@@ -114,7 +114,7 @@ InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z) {
     // the pass reads whole 16-byte chunks around each row segment: keep
     // 64 bytes of slack after the tensors it reads
     L.off_S = take(xyz * es + 64);
-    L.off_S0 = take(xyz * es + 64);
+    L.off_S0 = with_S0 ? take(xyz * es + 64) : L.off_S;
     L.off_X = take((size_t)x * y * es);
     L.off_alpha = take((size_t)x * 8);
     L.off_beta = take((size_t)z * 8);

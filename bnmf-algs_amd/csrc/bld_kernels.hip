@@ -38,6 +38,7 @@
 
 #include "bld_types.hpp"
 #include "digamma.hpp"
+#include "philox.hpp"
 
 namespace bnmf {
 
@@ -895,6 +896,45 @@ hipError_t launch_score(dim3 grid, hipStream_t st, const BldInst* insts, const P
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bld_score_final_kernel, dim3(n_inst < 1024 ? n_inst : 1024), dim3(256), 0,
                        st, insts, n_inst);
+    return hipGetLastError();
+}
+
+// init_S on the device (bld_mult_funcs.hpp:36-74): one thread per fiber,
+// the same Philox stream as the host init_S. The Exp(1) draws are made twice
+// (once for their sum, once for the output) instead of being stored, so no
+// per-thread array is needed; the values agree with the host's to the last
+// bit of the double log.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    bld_init_s_kernel(T* __restrict__ S, const T* __restrict__ X, size_t n_fibers, uint32_t z,
+                      uint64_t seed) {
+    const size_t f = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (f >= n_fibers) return;
+    const double xv = (double)X[f];
+    T* out = S + f * z;
+    if (xv == 0.0) {
+        for (uint32_t k = 0; k < z; ++k) out[k] = T(0);
+        return;
+    }
+    double tot = 0.0;
+    {
+        Philox g(seed, ST_INIT, f);
+        for (uint32_t k = 0; k < z; ++k) tot += -log(g.uniform());
+    }
+    Philox g(seed, ST_INIT, f);
+    for (uint32_t k = 0; k < z; ++k) out[k] = (T)(xv * (-log(g.uniform()) / tot));
+}
+
+hipError_t launch_init_s(int dtype, void* S, const void* X, size_t n_fibers, uint32_t z,
+                         uint64_t seed, hipStream_t st) {
+    if (n_fibers == 0) return hipSuccess;
+    const dim3 grid((unsigned)((n_fibers + 255) / 256));
+    if (dtype == 0)
+        hipLaunchKernelGGL(bld_init_s_kernel<float>, grid, dim3(256), 0, st, (float*)S,
+                           (const float*)X, n_fibers, z, seed);
+    else
+        hipLaunchKernelGGL(bld_init_s_kernel<double>, grid, dim3(256), 0, st, (double*)S,
+                           (const double*)X, n_fibers, z, seed);
     return hipGetLastError();
 }
 

@@ -48,6 +48,8 @@ hipError_t launch_prep(int dtype, int psi, bool raw, dim3 grid, hipStream_t st,
 hipError_t launch_score(dim3 grid, hipStream_t st, const BldInst* insts, const PrepItem* items,
                         const uint32_t* n_items, uint32_t n_inst);
 hipError_t launch_psi_eval(int kind, const double* xin, double* out, size_t n, hipStream_t st);
+hipError_t launch_init_s(int dtype, void* S, const void* X, size_t n_fibers, uint32_t z,
+                         uint64_t seed, hipStream_t st);
 hipError_t init_kernel_attributes();
 void ensure_runtime();  // per-process kernel attributes, once
 
@@ -104,7 +106,9 @@ struct InstLayout {
     size_t bytes;  // total, 256-byte aligned
     size_t elements() const { return (size_t)x * y * z; }
 };
-InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z);
+// with_S0: reserve a separate buffer for the initial tensor (batches keep S0
+// to restart solves; the server solves in place).
+InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z, bool with_S0 = true);
 BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first,
                   bool nonneg = false);
 

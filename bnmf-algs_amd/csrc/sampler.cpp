@@ -21,61 +21,15 @@
 #include <vector>
 
 #include "bnmf_internal.hpp"
+#include "philox.hpp"
 
 namespace bnmf {
 namespace {
 
-struct Philox {
-    uint32_t k0, k1;
-    uint32_t c0, c1, c2, c3;
-    uint32_t buf[4];
-    int pos = 4;
-
-    Philox(uint64_t seed, uint32_t stream, uint64_t index)
-        : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), c0(0), c1(stream),
-          c2((uint32_t)index), c3((uint32_t)(index >> 32)) {}
-
-    static inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
-        const uint64_t p = (uint64_t)a * b;
-        hi = (uint32_t)(p >> 32);
-        lo = (uint32_t)p;
-    }
-    void block() {
-        uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, a = k0, b = k1;
-        for (int r = 0; r < 10; ++r) {
-            uint32_t hi0, lo0, hi1, lo1;
-            mulhilo(0xD2511F53u, x0, hi0, lo0);
-            mulhilo(0xCD9E8D57u, x2, hi1, lo1);
-            const uint32_t n0 = hi1 ^ x1 ^ a, n2 = hi0 ^ x3 ^ b;
-            x0 = n0;
-            x1 = lo1;
-            x2 = n2;
-            x3 = lo0;
-            a += 0x9E3779B9u;
-            b += 0xBB67AE85u;
-        }
-        buf[0] = x0;
-        buf[1] = x1;
-        buf[2] = x2;
-        buf[3] = x3;
-        ++c0;  // next block of this variate's stream
-        pos = 0;
-    }
-    uint32_t next32() {
-        if (pos == 4) block();
-        return buf[pos++];
-    }
-    // uniform on the open interval (0, 1), 53 random bits
-    double uniform() {
-        const uint64_t hi = next32(), lo = next32();
-        const uint64_t m = ((hi << 21) ^ (lo >> 11)) & ((1ull << 53) - 1);
-        return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
-    }
-    double normal() {
-        const double u1 = uniform(), u2 = uniform();
-        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-    }
-};
+double normal_draw(Philox& g) {  // Box-Muller
+    const double u1 = g.uniform(), u2 = g.uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
 
 double gamma_draw(Philox& g, double shape, double scale) {
     if (shape < 1.0) {
@@ -86,7 +40,7 @@ double gamma_draw(Philox& g, double shape, double scale) {
     for (;;) {
         double xn, v;
         do {
-            xn = g.normal();
+            xn = normal_draw(g);
             v = 1.0 + c * xn;
         } while (v <= 0.0);
         v = v * v * v;
@@ -127,9 +81,6 @@ double poisson_draw(Philox& g, double mu) {
             return k;
     }
 }
-
-// stream ids: one per kind of variate
-enum : uint32_t { ST_L = 1, ST_W = 2, ST_H = 3, ST_X = 4, ST_S = 5, ST_INIT = 6 };
 
 template <typename F>
 void parallel_for(size_t n, F&& f) {
@@ -217,26 +168,9 @@ int sample_S(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
 template <typename T>
 int init_S(uint64_t seed, const T* X, size_t x, size_t y, size_t z, T* S) {
     parallel_for(x * y, [&](size_t f) {
-        T* s = S + f * z;
-        const double xv = (double)X[f];
-        if (xv == 0.0) {
-            for (size_t k = 0; k < z; ++k) s[k] = T(0);
-            return;
-        }
-        Philox g(seed, ST_INIT, f);
-        double tot = 0.0;
-        double v[256];
-        std::vector<double> big;
-        double* vp = v;
-        if (z > 256) {
-            big.resize(z);
-            vp = big.data();
-        }
-        for (size_t k = 0; k < z; ++k) {
-            vp[k] = -std::log(g.uniform());  // Gamma(1, 1)
-            tot += vp[k];
-        }
-        for (size_t k = 0; k < z; ++k) s[k] = (T)(xv * (vp[k] / tot));
+        thread_local std::vector<double> w;
+        if (w.size() < z) w.resize(z);
+        init_fiber<T>(seed, f, (double)X[f], (uint32_t)z, S + f * z, w.data());
     });
     return 0;
 }

@@ -123,6 +123,49 @@ bnmf_status bnmf_batch_get_info(bnmf_batch* b, bnmf_batch_info* info);
 void bnmf_batch_destroy(bnmf_batch* b);
 
 /* ------------------------------------------------------------------------
+ * Serving: a continuous-batching solve server. Requests are admitted in
+ * arrival order while their HBM footprint (S plus the per-instance
+ * workspace, bnmf_server_request_bytes) fits in the budget; a request larger
+ * than the whole budget is rejected (BNMF_ERR_REJECTED from collect). Active
+ * requests advance together; new ones join at the next scheduling step.
+ * S stays in HBM until collected. When a request has no S0 it is drawn on
+ * the GPU from the same stream as bnmf_init_S (seed). The request's host
+ * buffers are copied at submit time. No counterpart in the reference.
+ * ---------------------------------------------------------------------- */
+typedef struct bnmf_server bnmf_server;
+
+typedef struct bnmf_request_stats {
+    double submit_s, admit_s, done_s; /* steady-clock seconds */
+    size_t iterations;
+    size_t device_bytes;
+} bnmf_request_stats;
+
+typedef struct bnmf_server_stats {
+    size_t submitted, completed, rejected, active, queued, steps;
+    size_t bytes_in_use, peak_bytes, peak_active;
+    double busy_s; /* wall time the scheduler spent in device steps */
+} bnmf_server_stats;
+
+bnmf_status bnmf_server_create(bnmf_server** out, int dtype, size_t hbm_budget_bytes,
+                               size_t max_active, int use_psi_appr, double eps);
+bnmf_status bnmf_server_submit(bnmf_server* s, const bnmf_bld_desc* req, size_t max_iter,
+                               int want_log_marginal, uint64_t* ticket);
+/* *done = 1 once the request has finished (or failed). */
+bnmf_status bnmf_server_poll(bnmf_server* s, uint64_t ticket, int* done);
+/* Waits up to timeout_s (< 0: no limit); BNMF_ERR_TIMEOUT if still running;
+ * otherwise the request's own status. */
+bnmf_status bnmf_server_wait(bnmf_server* s, uint64_t ticket, double timeout_s);
+/* Copies S (x*y*z) and the FP64 log-marginal (NaN unless requested) of a
+ * finished request, frees its HBM and forgets the ticket. Any output may be
+ * NULL. */
+bnmf_status bnmf_server_collect(bnmf_server* s, uint64_t ticket, void* S_out,
+                                double* log_marginal_out, bnmf_request_stats* stats);
+bnmf_status bnmf_server_get_stats(bnmf_server* s, bnmf_server_stats* out);
+/* HBM bytes a request of this shape holds while admitted (0 if z > 64). */
+size_t bnmf_server_request_bytes(int dtype, size_t x, size_t y, size_t z);
+void bnmf_server_destroy(bnmf_server* s);
+
+/* ------------------------------------------------------------------------
  * Input synthesis (host): the allocation model of bnmf_priors / sample_S
  * (src/alloc_model/alloc_model_funcs.hpp:463-552) from a counter-based
  * Philox4x32-10 stream, so the output does not depend on the thread count.
