@@ -193,6 +193,17 @@ def serving(hip, n_requests, rate_frac, log, iters=200, pool_size=48):
     return res
 
 
+def measured_traffic():
+    """HBM bytes per pass of the headline workload from the committed rocprof
+    FETCH_SIZE/WRITE_SIZE runs (profiles/bld_pass_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "bld_pass_traffic.json")
+    try:
+        with open(path) as fh:
+            return float(json.load(fh)["traffic_bytes_per_pass"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(hip):
     from oracle import oracle
     inst = single(hip, 100, 100, 17, 40.0, 0xB1D0)
@@ -226,7 +237,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-loop", action="store_true")
     ap.add_argument("--only-headline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 1 GiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 256 << 20
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(instances would shard across GPUs with no communication).", file=sys.stderr)
@@ -356,7 +379,7 @@ def main():
                    "pass_items": info["pass_items"], "z_buckets": info["n_buckets"],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "bld_pass_kernel (FP32 update), all z buckets of one iteration",
                      "pass_us": pass_ms * 1e3, "iter_us": iter_ms * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes},
