@@ -1,0 +1,10 @@
+// bnmf_algs.hpp — umbrella header of the drop-in C++ API (the path of
+// src/bnmf_algs.hpp that this library provides: bld_mult and its helpers).
+#pragma once
+
+#include "alloc_model/alloc_model_funcs.hpp"
+#include "alloc_model/alloc_model_params.hpp"
+#include "bld/bld_fact.hpp"
+#include "bld/bld_mult.hpp"
+#include "defs.hpp"
+#include "util/util.hpp"
