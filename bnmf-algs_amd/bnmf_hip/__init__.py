@@ -116,6 +116,17 @@ def lib():
         L.bnmf_init_S.restype = st
         L.bnmf_psi_eval.argtypes = [ctypes.c_int, _D, _D, _SZ]
         L.bnmf_psi_eval.restype = st
+        L.bnmf_tune_set.argtypes = [ctypes.c_int, _SZ, _SZ, ctypes.c_uint]
+        L.bnmf_tune_set.restype = st
+        L.bnmf_tune_get.argtypes = [ctypes.c_int, _SZ, _SZ, ctypes.POINTER(ctypes.c_uint)]
+        L.bnmf_tune_get.restype = st
+        L.bnmf_tune_load.argtypes = [ctypes.c_char_p]
+        L.bnmf_tune_load.restype = st
+        L.bnmf_tune_save.argtypes = [ctypes.c_char_p]
+        L.bnmf_tune_save.restype = st
+        L.bnmf_tile_rows.argtypes = [ctypes.c_int, _SZ, _SZ, _SZ]
+        L.bnmf_tile_rows.restype = ctypes.c_uint
+        L.bnmf_tune_source.restype = ctypes.c_char_p
         if hasattr(L, "bnmf_server_create"):
             L.bnmf_server_create.argtypes = [ctypes.POINTER(_VP), ctypes.c_int, _SZ, _SZ,
                                              ctypes.c_int, ctypes.c_double]
@@ -474,3 +485,89 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+# --------------------------------------------------------------- autotuner
+# Shape classes of the tile table (include/bnmf_hip.h): z classes end at
+# these depths; size classes split fibers = x*y at 2^14, 2^16, 2^18, 2^20.
+TUNE_Z_CLASSES = (4, 8, 12, 16, 24, 32, 48, 64)
+TUNE_FIBER_CLASSES = ((1 << 10, 1 << 14), (1 << 14, 1 << 16), (1 << 16, 1 << 18),
+                      (1 << 18, 1 << 20), (1 << 20, 1 << 22))
+
+
+def tune_set(dtype, z: int, fibers: int, rows: int):
+    """Sets the tile height of the shape class of (z, fibers); 0 = rule."""
+    _check(lib().bnmf_tune_set(_code(_np_dtype(dtype)), int(z), int(fibers), int(rows)), "tune_set")
+
+
+def tune_get(dtype, z: int, fibers: int) -> int:
+    r = ctypes.c_uint(0)
+    _check(lib().bnmf_tune_get(_code(_np_dtype(dtype)), int(z), int(fibers), ctypes.byref(r)),
+           "tune_get")
+    return int(r.value)
+
+
+def tune_load(path: str):
+    _check(lib().bnmf_tune_load(os.fsencode(path)), "tune_load")
+
+
+def tune_save(path: str):
+    _check(lib().bnmf_tune_save(os.fsencode(path)), "tune_save")
+
+
+def tune_source() -> str:
+    return lib().bnmf_tune_source().decode()
+
+
+def tile_rows(dtype, x: int, y: int, z: int) -> int:
+    """Rows per pass tile an instance of this shape gets (table or rule)."""
+    return int(lib().bnmf_tile_rows(_code(_np_dtype(dtype)), int(x), int(y), int(z)))
+
+
+def autotune(dtype=np.float32, z_classes=TUNE_Z_CLASSES, fiber_classes=TUNE_FIBER_CLASSES,
+             candidates=(16, 32, 64, 128, 256), reps: int = 10, target_elements: int = 1 << 26,
+             max_instances: int = 256, path: Optional[str] = None, seed: int = 0x7A7E,
+             log=None) -> dict:
+    """Times the fused pass for each tile height candidate on one
+    representative workload per shape class and keeps the fastest in the
+    tile table (saved to `path` if given).
+
+    A class's workload is n instances of x = y = sqrt(geometric mid of the
+    fiber range) and z = the class's largest depth, with n chosen so the
+    batch holds about `target_elements` tensor elements (small instances
+    are solved batched). X comes from the allocation model (a = 40, b = 1).
+    Candidates that give the same tiling as an earlier one are skipped.
+    Returns {(zc, fc): {"rows": best, "pass_ms": {rows: ms}}}."""
+    dt = _np_dtype(dtype)
+    rng = np.random.default_rng(seed)
+    results = {}
+    for zc, z in enumerate(z_classes):
+        for fc, (lo, hi) in enumerate(fiber_classes):
+            fib = int(np.sqrt(lo * (hi - 1)))
+            side = max(1, int(round(np.sqrt(fib))))
+            n = int(min(max_instances, max(1, target_elements // (side * side * z))))
+            p = Params(40.0, 1.0, [1.0] * side, [1.0] * z)
+            insts = []
+            for t in range(n):
+                X = sample_X(int(rng.integers(1 << 62)), (side, side, z), p).astype(dt)
+                insts.append(Instance(X=X, z=z, params=p, seed=int(rng.integers(1 << 62))))
+            timed, seen = {}, {}
+            for cand in candidates:
+                tune_set(dt, z, side * side, cand)
+                rows = tile_rows(dt, side, side, z)
+                if rows in seen:
+                    continue
+                seen[rows] = cand
+                with Batch(insts, dtype=dt, use_psi_appr=True) as b:
+                    b.solve(2)
+                    timed[cand] = b.time_pass(reps)[0]
+            best = min(timed, key=timed.get)
+            tune_set(dt, z, side * side, best)
+            results[(zc, fc)] = {"rows": best, "pass_ms": timed, "shape": (side, side, z),
+                                 "instances": n}
+            if log:
+                log(f"[autotune] z<={z} fibers~{side * side} x{n}: " +
+                    " ".join(f"{c}:{t * 1e3:.0f}us" for c, t in timed.items()) + f" -> {best}")
+    if path:
+        tune_save(path)
+    return results

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -31,7 +32,10 @@ uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 void ensure_runtime() {
     static std::once_flag once;
     static hipError_t status = hipSuccess;
-    std::call_once(once, [] { status = init_kernel_attributes(); });
+    std::call_once(once, [] {
+        status = init_kernel_attributes();
+        if (status == hipSuccess) status = init_f32z_attributes();
+    });
     BNMF_HIP(status);
 }
 
@@ -61,8 +65,6 @@ uint32_t z_bucket(uint32_t z) {
 }
 
 uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
-    (void)dtype;
-    (void)z;
     // Row-tile height depends on the instance shape only (never on the rest
     // of a batch), so an instance sums in the same order alone or batched.
     // Tall tiles (up to 128 rows) amortise the tile prologue/epilogue and
@@ -78,8 +80,10 @@ uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
         return e ? std::strtol(e, nullptr, 10) : 0L;
     }();
     const bool large = (uint64_t)x * y >= (1ull << 20);
+    // the autotuner's table entry for this shape class, if any
+    const uint32_t tuned = tune_ti_cap(dtype, z, (uint64_t)x * y);
     const uint32_t cap = env_cap > 0 ? (uint32_t)std::min<long>(std::max<long>(env_cap, 4), 1024)
-                                     : (large ? 128u : 64u);
+                                     : tuned ? tuned : (large ? 128u : 64u);
     uint32_t nt = (x + cap - 1) / cap;
     if (large) {
         const uint32_t fill = (512 + n_tj - 1) / n_tj;
@@ -172,6 +176,33 @@ BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double 
     return d;
 }
 
+PassKey pass_key(int dtype, int mode, const InstLayout& L, bool nonneg) {
+    // BNMF_PASS_GENERIC=1 routes the FP32 update through the generic
+    // z-bucket kernel (same arithmetic; for comparisons)
+    static const bool generic = [] {
+        const char* e = std::getenv("BNMF_PASS_GENERIC");
+        return e != nullptr && e[0] == '1';
+    }();
+    PassKey k;
+    if (dtype == BNMF_F32 && mode == MODE_UPDATE && nonneg && !generic) {
+        k.zexact = (int)L.z;
+        return k;
+    }
+    k.zb = (int)L.zb;
+    k.rw = mode == MODE_UPDATE ? (int)L.rw : 1;
+    return k;
+}
+
+hipError_t launch_pass_key(const PassKey& k, int dtype, int psi, int mode, dim3 grid,
+                           hipStream_t st, const BldInst* insts, const PassItem* items,
+                           const uint32_t* n_items, uint32_t zmax) {
+    if (k.zexact > 0) {
+        if (dtype != BNMF_F32 || mode != MODE_UPDATE) return hipErrorInvalidValue;
+        return launch_pass_f32z(k.zexact, grid, st, insts, items, n_items);
+    }
+    return launch_pass(dtype, psi, mode, k.zb, k.rw, grid, st, insts, items, n_items, zmax);
+}
+
 // ------------------------------------------------------------------ batch
 Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& specs)
     : dtype_(dtype), psi_(psi_mode), eps_(eps), es_(dtype == BNMF_F32 ? 4 : 8), specs_(specs) {
@@ -238,27 +269,27 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     // descriptors and work lists
     std::vector<BldInst> d_first, d_main;
     std::vector<PrepItem> prep;
-    // one pass launch per (z bucket, LDS read width) group
-    std::vector<std::vector<PassItem>> pass_b(12);
-    std::vector<uint32_t> zmax_b(12, 1);
-    auto bidx = [](uint32_t zb, uint32_t rw) {
-        const int zi = zb == 8 ? 0 : zb == 16 ? 1 : zb == 32 ? 2 : 3;
-        const int ri = rw == 1 ? 0 : rw == 2 ? 1 : 2;
-        return zi * 3 + ri;
-    };
     for (size_t n = 0; n < specs.size(); ++n) {
         const InstLayout& L = layouts_[n];
         char* b = arena_.as<char>(base_off[n]);
         d_first.push_back(make_desc(L, b, eps, specs[n].a, specs[n].b, true, nonneg[n]));
         d_main.push_back(make_desc(L, b, eps, specs[n].a, specs[n].b, false, nonneg[n]));
         if (L.elements() == 0) continue;
-        const int bi = bidx(L.zb, L.rw);
-        zmax_b[bi] = std::max(zmax_b[bi], L.z);
-        for (uint32_t ti = 0; ti < L.n_ti; ++ti)
-            for (uint32_t tj = 0; tj < L.n_tj; ++tj)
-                pass_b[bi].push_back(PassItem{(uint32_t)n, ti, tj, 0});
         for (uint32_t c = 0; c < L.n_cj; ++c) prep.push_back(PrepItem{(uint32_t)n, 0, c, 0});
         for (uint32_t c = 0; c < L.nA; ++c) prep.push_back(PrepItem{(uint32_t)n, 1, c, 0});
+    }
+    // one pass launch per kernel key, separately for the update pass and for
+    // the sums-only / scoring passes
+    std::map<PassKey, std::vector<PassItem>> items_of[2];
+    for (int g = 0; g < 2; ++g) {
+        const int mode = g == 0 ? MODE_UPDATE : MODE_SUMS;
+        for (size_t n = 0; n < specs.size(); ++n) {
+            const InstLayout& L = layouts_[n];
+            if (L.elements() == 0) continue;
+            auto& v = items_of[g][pass_key(dtype, mode, L, nonneg[n])];
+            for (uint32_t ti = 0; ti < L.n_ti; ++ti)
+                for (uint32_t tj = 0; tj < L.n_tj; ++tj) v.push_back(PassItem{(uint32_t)n, ti, tj, 0});
+        }
     }
     size_t moff = 0;
     auto mtake = [&](size_t nbytes) {
@@ -271,25 +302,25 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     off_prep_items_ = mtake(prep.size() * sizeof(PrepItem));
     off_prep_count_ = mtake(sizeof(uint32_t));
     n_prep_items_ = (uint32_t)prep.size();
-    const int zbs[4] = {8, 16, 32, 64}, rws[3] = {1, 2, 4};
-    for (int bi = 0; bi < 12; ++bi) {
-        if (pass_b[bi].empty()) continue;
-        Bucket bk;
-        bk.zb = zbs[bi / 3];
-        bk.rw = rws[bi % 3];
-        bk.zmax = zmax_b[bi];
-        bk.work = 0;
-        for (const PassItem& pi : pass_b[bi]) {
-            const InstLayout& L = layouts_[pi.inst];
-            const uint32_t rows = std::min(L.ti_rows, L.x - pi.ti * L.ti_rows);
-            const uint32_t cols = std::min(kTileCols, L.y - pi.tj * kTileCols);
-            bk.work += (size_t)rows * cols * L.z;
+    for (int g = 0; g < 2; ++g) {
+        for (auto& kv : items_of[g]) {
+            Bucket bk;
+            bk.key = kv.first;
+            bk.zmax = 1;
+            bk.work = 0;
+            for (const PassItem& pi : kv.second) {
+                const InstLayout& L = layouts_[pi.inst];
+                const uint32_t rows = std::min(L.ti_rows, L.x - pi.ti * L.ti_rows);
+                const uint32_t cols = std::min(kTileCols, L.y - pi.tj * kTileCols);
+                bk.work += (size_t)rows * cols * L.z;
+                bk.zmax = std::max(bk.zmax, L.z);
+            }
+            bk.lane = 0;
+            bk.n_items = (uint32_t)kv.second.size();
+            bk.off_items = mtake(kv.second.size() * sizeof(PassItem));
+            bk.off_count = mtake(sizeof(uint32_t));
+            groups_[g].push_back(bk);
         }
-        bk.lane = 0;
-        bk.n_items = (uint32_t)pass_b[bi].size();
-        bk.off_items = mtake(pass_b[bi].size() * sizeof(PassItem));
-        bk.off_count = mtake(sizeof(uint32_t));
-        buckets_.push_back(bk);
     }
     meta_ = DeviceBuffer(std::max<size_t>(moff, kAlign));
     device_bytes_ += meta_.bytes;
@@ -301,26 +332,30 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     if (!prep.empty())
         BNMF_HIP(hipMemcpy(m + off_prep_items_, prep.data(), prep.size() * sizeof(PrepItem), hipMemcpyHostToDevice));
     BNMF_HIP(hipMemcpy(m + off_prep_count_, &n_prep_items_, sizeof(uint32_t), hipMemcpyHostToDevice));
-    size_t bpos = 0;
-    for (int bi = 0; bi < 12; ++bi) {
-        if (pass_b[bi].empty()) continue;
-        Bucket& bk = buckets_[bpos++];
-        BNMF_HIP(hipMemcpy(m + bk.off_items, pass_b[bi].data(), pass_b[bi].size() * sizeof(PassItem), hipMemcpyHostToDevice));
-        BNMF_HIP(hipMemcpy(m + bk.off_count, &bk.n_items, sizeof(uint32_t), hipMemcpyHostToDevice));
+    for (int g = 0; g < 2; ++g) {
+        size_t bpos = 0;
+        for (auto& kv : items_of[g]) {
+            Bucket& bk = groups_[g][bpos++];
+            BNMF_HIP(hipMemcpy(m + bk.off_items, kv.second.data(), kv.second.size() * sizeof(PassItem), hipMemcpyHostToDevice));
+            BNMF_HIP(hipMemcpy(m + bk.off_count, &bk.n_items, sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
     }
     BNMF_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    // spread the bucket passes over up to four streams, largest work first
-    // onto the least loaded stream
-    const int ns = (int)std::min<size_t>(4, buckets_.size());
+    // spread each pass's launches over up to four streams, largest work
+    // first onto the least loaded stream
+    const int ns = (int)std::min<size_t>(4, std::max(groups_[0].size(), groups_[1].size()));
     if (ns > 1) {
-        std::vector<size_t> order(buckets_.size()), load(ns, 0);
-        for (size_t b = 0; b < order.size(); ++b) order[b] = b;
-        std::sort(order.begin(), order.end(),
-                  [&](size_t a, size_t b) { return buckets_[a].work > buckets_[b].work; });
-        for (size_t b : order) {
-            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            buckets_[b].lane = l;
-            load[l] += buckets_[b].work;
+        for (int g = 0; g < 2; ++g) {
+            std::vector<Bucket>& gs = groups_[g];
+            std::vector<size_t> order(gs.size()), load(ns, 0);
+            for (size_t b = 0; b < order.size(); ++b) order[b] = b;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](size_t a, size_t b) { return gs[a].work > gs[b].work; });
+            for (size_t b : order) {
+                const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                gs[b].lane = l;
+                load[l] += gs[b].work;
+            }
         }
         side_.resize(ns);
         fork_ev_.resize(ns);
@@ -352,12 +387,12 @@ void Batch::enqueue_pass(int mode, bool first) {
         BNMF_HIP(hipEventRecord(fork_ev_[0], stream_));
         for (size_t l = 1; l < side_.size(); ++l) BNMF_HIP(hipStreamWaitEvent(side_[l], fork_ev_[0], 0));
     }
-    for (const Bucket& bk : buckets_) {
+    for (const Bucket& bk : groups_of(mode)) {
         const dim3 grid(bk.n_items);
         hipStream_t st = fork ? side_[bk.lane] : stream_;
-        BNMF_HIP(launch_pass(dtype_, psi_, mode, bk.zb, bk.rw, grid, st, d,
-                             reinterpret_cast<const PassItem*>(m + bk.off_items),
-                             reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
+        BNMF_HIP(launch_pass_key(bk.key, dtype_, psi_, mode, grid, st, d,
+                                 reinterpret_cast<const PassItem*>(m + bk.off_items),
+                                 reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
     }
     if (fork) {
         for (size_t l = 1; l < side_.size(); ++l) {
@@ -516,10 +551,10 @@ void Batch::tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp) {
     DeviceBuffer dd(sizeof(BldInst));
     BNMF_HIP(hipMemcpy(dd.ptr, &d, sizeof(BldInst), hipMemcpyHostToDevice));
     char* m = meta_.as<char>();
-    const Bucket& bk = buckets_.at(0);
-    BNMF_HIP(launch_pass(dtype_, psi_, MODE_SUMS, bk.zb, bk.rw, dim3(bk.n_items), stream_,
-                         dd.as<BldInst>(), reinterpret_cast<const PassItem*>(m + bk.off_items),
-                         reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
+    const Bucket& bk = groups_of(MODE_SUMS).at(0);
+    BNMF_HIP(launch_pass_key(bk.key, dtype_, psi_, MODE_SUMS, dim3(bk.n_items), stream_,
+                             dd.as<BldInst>(), reinterpret_cast<const PassItem*>(m + bk.off_items),
+                             reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
     BNMF_HIP(launch_prep(dtype_, psi_, true, dim3(n_prep_items_), stream_, dd.as<BldInst>(),
                          reinterpret_cast<const PrepItem*>(m + off_prep_items_),
                          reinterpret_cast<const uint32_t*>(m + off_prep_count_)));
@@ -537,9 +572,9 @@ bnmf_batch_info Batch::info() const {
         in.fibers += (size_t)L.x * L.y;
     }
     in.device_bytes = device_bytes_;
-    for (const Bucket& bk : buckets_) in.pass_items += bk.n_items;
+    for (const Bucket& bk : groups_[0]) in.pass_items += bk.n_items;
     in.prep_items = n_prep_items_;
-    in.n_buckets = (int)buckets_.size();
+    in.n_buckets = (int)groups_[0].size();
     return in;
 }
 

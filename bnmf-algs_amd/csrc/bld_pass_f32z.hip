@@ -1,0 +1,399 @@
+// bld_pass_f32z.hip — the FP32-bulk update pass specialised on the exact
+// fiber depth z (1..64).
+//
+// Same work decomposition, LDS layout and arithmetic as the generic update
+// pass in bld_kernels.hip (a block of WAVES wavefronts per tile of ti_rows x
+// 64 fibers; each lane owns one fiber of the current row, held in a
+// wave-private LDS row buffer), but with z a compile-time constant: the
+// chunk loops have no per-chunk guards, the LDS read width and the XOR visit
+// order are constants, the per-fiber arrays have exactly z entries (fewer
+// registers, so more waves per SIMD for small z) and, for z <= 24, the fiber
+// stays in registers between the two phases instead of being read again.
+// It serves instances whose S stays nonnegative (S0 >= 0 and beta >= 1, the
+// descriptor's nonneg flag), so psi(S+1) always takes its x >= 1 form; other
+// instances run the generic pass. Results are bitwise identical to the
+// generic pass (same operations in the same order).
+//
+// Reference operations (src/bld/bld_mult/bld_mult_funcs.hpp): grad_plus
+// :177-192, nom :242-259, denom :272-290, update_S :307-327; the axis sums of
+// the new S (bld_mult.hpp:124-133) leave as per-tile partials.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bld_types.hpp"
+#include "digamma.hpp"
+#include "pass_common.hpp"
+
+namespace bnmf {
+
+// Compile-time form of read_layout() for float: LDS read width RW and the XOR
+// mask of the chunk visit order (lane l visits chunk m as m ^ ((l >> HS) & HM)).
+template <int Z>
+struct F32Layout {
+    static constexpr int RW = (Z % 4 == 0) ? 4 : ((Z % 2 == 0) ? 2 : 1);
+    static constexpr int NCH = Z / RW;
+    static constexpr int LPC = 128 / (RW * 4);
+    static constexpr int p_of() {
+        int p = 1;
+        if (RW == 1) return 1;
+        while (p < LPC && (NCH % (p * 2)) == 0) p *= 2;
+        return p;
+    }
+    static constexpr int P = p_of();
+    static constexpr int hs_of() {
+        int s = 0;
+        while ((1 << s) < LPC / P) ++s;
+        return s;
+    }
+    static constexpr int HS = hs_of();
+    static constexpr int HM = P - 1;
+    static constexpr int zq_of() {
+        int q = 1;
+        while (q < Z) q *= 2;
+        return q;
+    }
+    static constexpr int ZQ = zq_of();
+    static constexpr int ZQL = (ZQ == 1 ? 0 : ZQ == 2 ? 1 : ZQ == 4 ? 2 : ZQ == 8 ? 3 : ZQ == 16 ? 4 : ZQ == 32 ? 5 : 6);
+};
+
+// LDS of a block: [psib 64*Z][psiA Z][WAVES x grad_minus Z][WAVES x 64 X_ij]
+// [WAVES x row buffer]. A row buffer holds CPL*64 16-byte chunks: every lane writes all of
+// its chunks each row (the ones past the row end hold repeated data), so the
+// prefetch has a fixed instruction count.
+template <int Z>
+struct F32zLds {
+    static constexpr uint32_t CPL = (64 * Z + 8 + 255) / 256;  // 16-byte chunks per lane
+    static constexpr size_t psiA_off = round16((size_t)64 * Z * 4);
+    static constexpr size_t gm_off = psiA_off + round16((size_t)Z * 4);
+    static constexpr size_t gm_stride = round16((size_t)Z * 4);
+    static constexpr size_t raw_stride = (size_t)CPL * 64 * 16;
+    static constexpr size_t xs_off(int waves) { return gm_off + waves * gm_stride; }
+    static constexpr size_t raw_off(int waves) { return xs_off(waves) + waves * 256; }
+    static constexpr size_t bytes(int waves) { return raw_off(waves) + waves * raw_stride; }
+};
+
+template <int Z>
+constexpr int f32z_waves() {
+    int w = 4;
+    while (w > 1 && F32zLds<Z>::bytes(w) > kLdsBytes) --w;
+    return w;
+}
+
+// Waves per SIMD the register allocation must leave room for.
+#ifndef BNMF_F32Z_WPE
+#define BNMF_F32Z_WPE(Z) ((Z) <= 10 ? 4 : (Z) <= 18 ? 3 : (Z) <= 31 ? 2 : 1)
+#endif
+
+// Memory pipelining: a wave issues the next row's loads (S chunks, X, psia)
+// right after moving the current row into LDS, and every global store of a
+// row is unconditional (lanes with nothing to write target the 64-byte
+// slack after S). The instruction count between a row's loads and their
+// first use is then fixed, so the wait for them leaves the previous row's
+// stores in flight instead of draining the whole memory queue.
+template <int Z, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BNMF_F32Z_WPE(Z))))
+    bld_pass_f32z_kernel(const BldInst* __restrict__ insts, const PassItem* __restrict__ items,
+                         const uint32_t* __restrict__ n_items_p) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    using Lds = F32zLds<Z>;
+    using Lay = F32Layout<Z>;
+    constexpr int RW = Lay::RW;
+    constexpr int NCH = Lay::NCH;
+    constexpr uint32_t NT = 64 * WAVES;
+    constexpr uint32_t VEC = 4;
+    constexpr int CPL = (int)Lds::CPL;
+    constexpr bool KEEP = Z <= 24;  // fiber kept in registers across the phases
+    constexpr uint32_t ZQ = Lay::ZQ;
+    constexpr uint32_t L = 64u / ZQ;
+    typedef float V16 __attribute__((ext_vector_type(4)));
+    typedef typename VecR<float, RW>::type VR;
+
+    const uint32_t n_items = *n_items_p;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    // wave index as a scalar: row loops and their branches stay uniform
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
+    const uint32_t hsw = RW > 1 ? ((lane >> Lay::HS) & (uint32_t)Lay::HM) : 0u;
+
+    float* const psib_l = reinterpret_cast<float*>(lds_raw);
+    float* const psiA_l = reinterpret_cast<float*>(lds_raw + Lds::psiA_off);
+    float* const gm_l = reinterpret_cast<float*>(lds_raw + Lds::gm_off + w * Lds::gm_stride);
+    float* const xs_l = reinterpret_cast<float*>(lds_raw + Lds::xs_off(WAVES) + w * 256);
+    unsigned char* const raw_base = lds_raw + Lds::raw_off(WAVES);
+    float* const raw = reinterpret_cast<float*>(raw_base + w * Lds::raw_stride);
+
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const PassItem item = items[it];
+        const BldInst* __restrict__ I = insts + item.inst;
+        const uint32_t x = I->x, y = I->y;
+        const uint32_t i0 = item.ti * I->ti_rows;
+        const uint32_t i1 = min(x, i0 + I->ti_rows);
+        const uint32_t j0 = item.tj * kTileCols;
+        const uint32_t nj = min(kTileCols, y - j0);
+        const uint32_t n_el = nj * Z;
+
+        const BNMF_GLOBAL float* __restrict__ Ssrc = gconst<float>(I->S_src);
+        BNMF_GLOBAL float* __restrict__ Sdst = gmut<float>(I->S);
+        // in the 64 bytes after S (16-byte aligned): target of the stores
+        // that have nothing to write
+        BNMF_GLOBAL float* __restrict__ trash =
+            (BNMF_GLOBAL float*)(((uintptr_t)(Sdst + (size_t)x * y * Z) + 15u) & ~(uintptr_t)15u) +
+            (lane & 1u) * 4u;
+        const BNMF_GLOBAL float* __restrict__ Xg = gconst<float>(I->X);
+        const BNMF_GLOBAL float* __restrict__ psia_g = gconst<float>(I->psia);
+        const float epsf = static_cast<float>(I->eps);
+        const uint32_t lane_x = min(lane, nj - 1u);  // clamped fiber of this lane
+        const uint32_t lane_z = min(lane, (uint32_t)Z - 1u);
+
+        {
+            const BNMF_GLOBAL float* __restrict__ psib_g = gconst<float>(I->psib) + (size_t)j0 * Z;
+            for (uint32_t e = tid; e < n_el; e += NT) psib_l[e] = psib_g[e];
+            if (tid < Z) {
+                // A_k = sum_i alpha_eph(i,k), the argument of psi in
+                // update_grad_minus (bld_mult_funcs.hpp:213-220)
+                const BNMF_GLOBAL float* __restrict__ pA = gconst<float>(I->partA);
+                SumAcc<float> A;
+                for (uint32_t c = 0; c < I->nA; ++c) A.add(pA[(size_t)c * Z + tid]);
+                psiA_l[tid] = psi_sel<PSI_F32, float>(A.get());
+            }
+            __syncthreads();
+        }
+
+        float acc[Z];
+#pragma unroll
+        for (int k = 0; k < Z; ++k) acc[k] = 0.0f;
+
+        const uint32_t kq = lane & (ZQ - 1u);
+        const uint32_t part = lane >> Lay::ZQL;
+
+        // prefetch registers of one row: S chunks from the aligned start
+        // (chunk indices clamped to the row's last chunk), X_ij of the
+        // lane's fiber and psi(alpha_eph)(i, lane)
+        V16 pf[CPL];
+        float pf_x = 0.0f, pf_pa = 0.0f;
+        uint32_t i = i0 + w;
+        uint32_t dl = 0, nch = 0;
+        // real = false: there is no next row; every lane loads the row's
+        // first chunk (one cache line) so the instruction count stays fixed
+        auto issue_row = [&](uint32_t row, bool real, uint32_t& d_out, uint32_t& n_out) {
+            const size_t s0 = ((size_t)row * y + j0) * Z;
+            const uint32_t d = (uint32_t)(s0 % VEC);
+            const uint32_t n = (d + n_el + VEC - 1) / VEC;
+            const BNMF_GLOBAL V16* src = (const BNMF_GLOBAL V16*)(Ssrc + (s0 - d));
+            const uint32_t last = real ? n - 1u : 0u;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) pf[c] = src[min((uint32_t)c * 64u + lane, last)];
+            pf_x = Xg[(size_t)row * y + j0 + lane_x];
+            pf_pa = psia_g[(size_t)row * Z + lane_z];
+            d_out = d;
+            n_out = n;
+        };
+        // One row of the tile. The first row is peeled off the loop so that
+        // at the loop head the previous row's stores always follow the
+        // current row's loads in issue order.
+        auto row_step = [&](uint32_t i) {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) reinterpret_cast<V16*>(raw)[(uint32_t)c * 64u + lane] = pf[c];
+            if (lane < (uint32_t)Z) gm_l[lane] = psiA_l[lane] - pf_pa;
+            // X_ij goes through LDS too: the prefetch register is then free
+            // before the next row's load, and the loop carries no copy that
+            // would have to wait for that load
+            xs_l[lane] = pf_x;
+            const uint32_t cur_dl = dl, cur_nch = nch;
+            const size_t cur_s0 = ((size_t)i * y + j0) * Z;
+            const uint32_t inext = i + WAVES;
+            const bool more = inext < i1;
+            issue_row(more ? inext : i, more, dl, nch);
+            wave_sync();
+            const float xv = lane < nj ? xs_l[lane] : 0.0f;
+            if (lane < nj) {
+                float* const fb = raw + cur_dl + lane * Z;
+                const float* const pbb = psib_l + lane * Z;
+                float gp[Z];
+                float sv_keep[KEEP ? Z : 1];
+                float sumS = 0.0f, nom = 0.0f, den = 0.0f;
+                // eps = 1e-50 flushes to zero in float; an X_ij = 0 or
+                // all-zero fiber updates to exactly 0 as in the reference's
+                // double-promoted guard.
+                const bool live = xv > 0.0f;
+                const float xr = live ? 1.0f / (xv + epsf) : 0.0f;
+                auto phase_a = [&](auto psi_of) {
+#pragma unroll
+                    for (int m = 0; m < NCH; ++m) {
+                        const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                        const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                        const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                        const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+#pragma unroll
+                        for (int v = 0; v < RW; ++v) {
+                            const float sk = vget<float, RW>(sv, v);
+                            if constexpr (KEEP) sv_keep[m * RW + v] = sk;
+                            const float g = vget<float, RW>(pb, v) - psi_of(sk);
+                            gp[m * RW + v] = g;
+                            sumS += sk;
+                            const float t = sk * xr;
+                            nom = fmaf(t, vget<float, RW>(gv, v), nom);
+                            den = fmaf(t, g, den);
+                        }
+                    }
+                };
+                // S >= 0 throughout (the instance has S0 >= 0 and beta >= 1),
+                // so s + 1 >= 1 and psi takes its x >= 1 form directly
+                phase_a([](float sk) { return psi_ge1_f32(sk + 1.0f); });
+                const bool upd = live && sumS != 0.0f;
+                const float x_over_s = upd ? xv / (sumS + epsf) : 0.0f;
+                const float den_e = den + epsf;
+#pragma unroll
+                for (int m = 0; m < NCH; ++m) {
+                    const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                    VR sv;
+                    if constexpr (KEEP) {
+#pragma unroll
+                        for (int v = 0; v < RW; ++v) vset<float, RW>(sv, v, sv_keep[m * RW + v]);
+                    } else {
+                        sv = *reinterpret_cast<const VR*>(fb + mm);
+                    }
+                    const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+#pragma unroll
+                    for (int v = 0; v < RW; ++v) {
+                        const float sk = vget<float, RW>(sv, v);
+                        const float fct = (gp[m * RW + v] + nom) * x_over_s *
+                                          recip_fast<float>(vget<float, RW>(gv, v) + den_e);
+                        const float sn = upd ? sk * fct : 0.0f;
+                        vset<float, RW>(sv, v, sn);
+                        acc[m * RW + v] += sn;
+                    }
+                    *reinterpret_cast<VR*>(fb + mm) = sv;
+                }
+            }
+            wave_sync();
+
+            // S_i+k partial over the tile's columns: eight independent
+            // partial sums combined in a fixed tree (deterministic)
+            {
+                float r[8];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) r[v] = 0.0f;
+                if (kq < (uint32_t)Z) {
+                    const float* const rb = raw + cur_dl + kq;
+                    for (uint32_t f0 = part; f0 < nj; f0 += 8u * L) {
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) {
+                            const uint32_t f = f0 + (uint32_t)v * L;
+                            if (f < nj) r[v] += rb[f * Z];
+                        }
+                    }
+                }
+                float rs = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+#pragma unroll
+                for (uint32_t off = ZQ; off < 64u; off <<= 1) rs += __shfl_xor(rs, (int)off);
+                if (part == 0 && kq < (uint32_t)Z)
+                    gmut<float>(I->part_ipk)[((size_t)item.tj * x + i) * Z + kq] = rs;
+            }
+
+            // the row back to S: 16-byte chunks that lie inside the row
+            // segment go to their place, the rest to the slack; then the (at
+            // most two) chunks straddling the segment ends element by
+            // element, lanes 0-3 the head chunk and lanes 4-7 the tail chunk
+            {
+                BNMF_GLOBAL V16* __restrict__ dst = (BNMF_GLOBAL V16*)(Sdst + (cur_s0 - cur_dl));
+                const uint32_t lo = cur_dl, hi = cur_dl + n_el;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const uint32_t q = (uint32_t)c * 64u + lane;
+                    const uint32_t e0 = q * VEC;
+                    const bool inside = q < cur_nch && e0 >= lo && e0 + VEC <= hi;
+                    BNMF_GLOBAL V16* to = inside ? dst + q : (BNMF_GLOBAL V16*)trash;
+                    *to = reinterpret_cast<const V16*>(raw)[q];
+                }
+                const uint32_t qe = lane < 4u ? 0u : cur_nch - 1u;
+                const uint32_t e = qe * VEC + (lane & 3u);
+                const bool mine = lane < 8u && e >= lo && e < hi &&
+                                  ((qe * VEC < lo) || (qe * VEC + VEC > hi));
+                BNMF_GLOBAL float* to = mine ? (BNMF_GLOBAL float*)dst + e : trash;
+                *to = raw[min(e, (uint32_t)Lds::raw_stride / 4u - 1u)];
+            }
+            wave_sync();
+        };
+        if (i < i1) {
+            issue_row(i, true, dl, nch);
+            row_step(i);
+            for (i += WAVES; i < i1; i += WAVES) row_step(i);
+        }
+
+        // S_+jk partial over the tile's rows: the waves' column sums meet in
+        // LDS (natural fiber layout) and are added in wave order
+        __syncthreads();
+        float* const comb_self = reinterpret_cast<float*>(raw_base + w * Lds::raw_stride);
+        if (lane < nj) {
+#pragma unroll
+            for (int m = 0; m < NCH; ++m) {
+                const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+#pragma unroll
+                for (int v = 0; v < RW; ++v) comb_self[lane * Z + mm + v] = acc[m * RW + v];
+            }
+        }
+        __syncthreads();
+        {
+            BNMF_GLOBAL float* __restrict__ pp = gmut<float>(I->part_pjk) + ((size_t)item.ti * y + j0) * Z;
+            for (uint32_t e = tid; e < n_el; e += NT) {
+                float sacc = 0.0f;
+#pragma unroll
+                for (int w2 = 0; w2 < WAVES; ++w2)
+                    sacc += reinterpret_cast<const float*>(raw_base + w2 * Lds::raw_stride)[e];
+                pp[e] = sacc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int Z>
+static hipError_t launch_f32z_t(dim3 grid, hipStream_t st, const BldInst* insts,
+                                const PassItem* items, const uint32_t* n_items) {
+    constexpr int WAVES = f32z_waves<Z>();
+    const size_t lds = F32zLds<Z>::bytes(WAVES);
+    hipLaunchKernelGGL((bld_pass_f32z_kernel<Z, WAVES>), grid, dim3(64 * WAVES), lds, st, insts,
+                       items, n_items);
+    return hipGetLastError();
+}
+
+template <int Z>
+static hipError_t attr_f32z_t() {
+    constexpr int WAVES = f32z_waves<Z>();
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(bld_pass_f32z_kernel<Z, WAVES>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)F32zLds<Z>::bytes(WAVES));
+}
+
+template <int... Zs>
+struct F32zTable {
+    static hipError_t launch(int z, dim3 grid, hipStream_t st, const BldInst* insts,
+                             const PassItem* items, const uint32_t* n_items) {
+        hipError_t e = hipErrorInvalidValue;
+        (void)((Zs == z ? (e = launch_f32z_t<Zs>(grid, st, insts, items, n_items), true) : false) || ...);
+        return e;
+    }
+    static hipError_t attrs() {
+        hipError_t e = hipSuccess;
+        (void)((e = attr_f32z_t<Zs>(), e != hipSuccess) || ...);
+        return e;
+    }
+};
+
+using F32zAll = F32zTable<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21,
+                          22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40,
+                          41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59,
+                          60, 61, 62, 63, 64>;
+
+// FP32 update pass of instances that all have depth z (1..64).
+hipError_t launch_pass_f32z(int z, dim3 grid, hipStream_t st, const BldInst* insts,
+                            const PassItem* items, const uint32_t* n_items) {
+    return F32zAll::launch(z, grid, st, insts, items, n_items);
+}
+
+hipError_t init_f32z_attributes() { return F32zAll::attrs(); }
+
+}  // namespace bnmf

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/bnmf_hip.h"
@@ -51,6 +52,10 @@ hipError_t launch_psi_eval(int kind, const double* xin, double* out, size_t n, h
 hipError_t launch_init_s(int dtype, void* S, const void* X, size_t n_fibers, uint32_t z,
                          uint64_t seed, hipStream_t st);
 hipError_t init_kernel_attributes();
+// FP32 update pass specialised on the exact fiber depth z (bld_pass_f32z.hip)
+hipError_t launch_pass_f32z(int z, dim3 grid, hipStream_t st, const BldInst* insts,
+                            const PassItem* items, const uint32_t* n_items);
+hipError_t init_f32z_attributes();
 void ensure_runtime();  // per-process kernel attributes, once
 
 // ----------------------------------------------------------------- sampler
@@ -66,6 +71,16 @@ int init_S(uint64_t seed, const T* X, size_t x, size_t y, size_t z, T* S);
 constexpr uint32_t kMaxZ = 64;
 uint32_t z_bucket(uint32_t z);
 uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z);
+
+// ------------------------------------------------------------- tile table
+// Rows per pass tile by shape class (tune.cpp); 0 = built-in rule.
+int tune_zclass(uint32_t z);
+int tune_sclass(uint64_t fibers);
+uint32_t tune_ti_cap(int dtype, uint32_t z, uint64_t fibers);
+void tune_set(int dtype, uint32_t z, uint64_t fibers, uint32_t cap);
+void tune_load(const std::string& path);
+void tune_save(const std::string& path);
+std::string tune_source();
 
 // --------------------------------------------------------- device buffers
 struct DeviceBuffer {
@@ -112,6 +127,22 @@ InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z, bool wit
 BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first,
                   bool nonneg = false);
 
+// Kernel of one pass launch: the exact-z FP32 update kernel (zexact > 0) or
+// the generic z-bucket kernel (zb, LDS read width rw). Instances with equal
+// keys share a launch. The key depends on the instance shape only, so an
+// instance runs the same kernel alone, batched or served.
+struct PassKey {
+    int zexact = 0, zb = 0, rw = 0;
+    bool operator<(const PassKey& o) const {
+        return std::tie(zexact, zb, rw) < std::tie(o.zexact, o.zb, o.rw);
+    }
+};
+// nonneg: the instance's S stays >= 0 (S0 >= 0 and beta >= 1)
+PassKey pass_key(int dtype, int mode, const InstLayout& L, bool nonneg);
+hipError_t launch_pass_key(const PassKey& k, int dtype, int psi, int mode, dim3 grid,
+                           hipStream_t st, const BldInst* insts, const PassItem* items,
+                           const uint32_t* n_items, uint32_t zmax);
+
 // A fixed set of instances sharing one arena, one stream and one set of
 // captured per-iteration graphs.
 class Batch {
@@ -129,7 +160,7 @@ class Batch {
 
    private:
     struct Bucket {
-        int zb, rw;
+        PassKey key;
         int lane;          // side stream the bucket's pass runs on
         size_t work;       // elements
         uint32_t zmax;
@@ -139,6 +170,9 @@ class Batch {
     };
     void enqueue_iteration(bool first);
     void enqueue_pass(int mode, bool first);
+    // pass launches of the update pass (groups_[0]) and of the sums-only and
+    // scoring passes (groups_[1])
+    std::vector<Bucket>& groups_of(int mode) { return groups_[mode == MODE_UPDATE ? 0 : 1]; }
     void capture_graphs();
 
     int dtype_, psi_;
@@ -151,7 +185,7 @@ class Batch {
     DeviceBuffer meta_;  // descriptors, work lists, counts
     size_t off_desc_first_ = 0, off_desc_ = 0, off_prep_items_ = 0, off_prep_count_ = 0;
     uint32_t n_prep_items_ = 0;
-    std::vector<Bucket> buckets_;
+    std::vector<Bucket> groups_[2];
     hipStream_t stream_ = nullptr;
     // z-bucket passes of one iteration are independent: they run on up to
     // four streams forked from and joined back to stream_ (graph branches

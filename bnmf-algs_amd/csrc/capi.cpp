@@ -306,4 +306,57 @@ bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n) {
     });
 }
 
+bnmf_status bnmf_tune_set(int dtype, size_t z, size_t fibers, unsigned rows) {
+    return guarded([&] {
+        check_dtype(dtype);
+        BNMF_REQUIRE(z >= 1 && z <= bnmf::kMaxZ, "z out of range");
+        BNMF_REQUIRE(rows == 0 || (rows >= 4 && rows <= 1024), "rows must be 0 or in [4, 1024]");
+        bnmf::tune_set(dtype, (uint32_t)z, (uint64_t)fibers, rows);
+    });
+}
+
+bnmf_status bnmf_tune_get(int dtype, size_t z, size_t fibers, unsigned* rows) {
+    return guarded([&] {
+        check_dtype(dtype);
+        BNMF_REQUIRE(rows != nullptr, "rows is null");
+        BNMF_REQUIRE(z >= 1 && z <= bnmf::kMaxZ, "z out of range");
+        *rows = bnmf::tune_ti_cap(dtype, (uint32_t)z, (uint64_t)fibers);
+    });
+}
+
+bnmf_status bnmf_tune_load(const char* path) {
+    return guarded([&] {
+        BNMF_REQUIRE(path != nullptr, "path is null");
+        bnmf::tune_load(path);
+    });
+}
+
+bnmf_status bnmf_tune_save(const char* path) {
+    return guarded([&] {
+        BNMF_REQUIRE(path != nullptr, "path is null");
+        bnmf::tune_save(path);
+    });
+}
+
+unsigned bnmf_tile_rows(int dtype, size_t x, size_t y, size_t z) {
+    if ((dtype != BNMF_F32 && dtype != BNMF_F64) || z < 1 || z > bnmf::kMaxZ || x == 0 || y == 0 ||
+        x >= (1u << 31) || y >= (1u << 31))
+        return 0;
+    try {
+        return bnmf::choose_ti_rows(dtype, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+    } catch (...) {
+        return 0;
+    }
+}
+
+const char* bnmf_tune_source(void) {
+    static thread_local std::string src;
+    try {
+        src = bnmf::tune_source();
+    } catch (...) {
+        src.clear();
+    }
+    return src.c_str();
+}
+
 }  // extern "C"

@@ -232,7 +232,8 @@ class Server {
 
    private:
     struct Launch {
-        int zb, rw, lane;
+        PassKey key;
+        int lane;
         uint32_t zmax, n;
         size_t off_items, off_count, work;
     };
@@ -310,9 +311,10 @@ class Server {
     }
 
 
-    // Builds descriptors and work lists for `set` in the meta buffer region
-    // starting at `moff`; returns the end offset.
-    size_t build(const std::vector<Request*>& set, WorkSet& ws, std::vector<char>& host, size_t moff) {
+    // Builds descriptors and the work lists of `mode` passes for `set` in the
+    // meta buffer region starting at `moff`; returns the end offset.
+    size_t build(const std::vector<Request*>& set, WorkSet& ws, std::vector<char>& host, size_t moff,
+                 int mode) {
         auto take = [&](size_t n) {
             const size_t o = moff;
             moff = align_up(moff + n);
@@ -321,16 +323,16 @@ class Server {
         };
         std::vector<BldInst> desc;
         std::vector<PrepItem> prep;
-        std::map<std::pair<int, int>, std::vector<PassItem>> groups;
-        std::map<std::pair<int, int>, uint32_t> zmax;
-        std::map<std::pair<int, int>, size_t> work;
+        std::map<PassKey, std::vector<PassItem>> groups;
+        std::map<PassKey, uint32_t> zmax;
+        std::map<PassKey, size_t> work;
         for (size_t n = 0; n < set.size(); ++n) {
             const Request& r = *set[n];
             const InstLayout& L = r.L;
             BldInst d = make_desc(L, dev_buf_.as<char>(r.off), eps_, r.a, r.b, false, r.nonneg);
             desc.push_back(d);
             if (L.elements() == 0) continue;
-            const auto key = std::make_pair((int)L.zb, (int)L.rw);
+            const PassKey key = pass_key(dtype_, mode, L, r.nonneg);
             zmax[key] = std::max(zmax[key], L.z);
             for (uint32_t ti = 0; ti < L.n_ti; ++ti)
                 for (uint32_t tj = 0; tj < L.n_tj; ++tj) groups[key].push_back(PassItem{(uint32_t)n, ti, tj, 0});
@@ -348,14 +350,13 @@ class Server {
         std::memcpy(host.data() + ws.off_prep_count, &ws.n_prep, 4);
         ws.launches.clear();
         std::vector<size_t> load(4, 0);
-        std::vector<std::pair<size_t, std::pair<int, int>>> order;
+        std::vector<std::pair<size_t, PassKey>> order;
         for (auto& g : groups) order.push_back({work[g.first], g.first});
-        std::sort(order.begin(), order.end(), [](auto& a, auto& b) { return a.first > b.first; });
+        std::stable_sort(order.begin(), order.end(), [](auto& a, auto& b) { return a.first > b.first; });
         for (auto& o : order) {
             auto& items = groups[o.second];
             Launch l;
-            l.zb = o.second.first;
-            l.rw = o.second.second;
+            l.key = o.second;
             l.zmax = zmax[o.second];
             l.n = (uint32_t)items.size();
             l.work = o.first;
@@ -386,9 +387,9 @@ class Server {
         }
         for (const Launch& l : ws.launches) {
             hipStream_t st = fork ? side_[l.lane] : stream_;
-            BNMF_HIP(launch_pass(dtype_, psi_, mode, l.zb, l.rw, dim3(l.n), st, d,
-                                 reinterpret_cast<const PassItem*>(m + l.off_items),
-                                 reinterpret_cast<const uint32_t*>(m + l.off_count), l.zmax));
+            BNMF_HIP(launch_pass_key(l.key, dtype_, psi_, mode, dim3(l.n), st, d,
+                                     reinterpret_cast<const PassItem*>(m + l.off_items),
+                                     reinterpret_cast<const uint32_t*>(m + l.off_count), l.zmax));
         }
         if (fork) {
             for (int l = 1; l < 4; ++l) {
@@ -409,8 +410,8 @@ class Server {
         }
         std::vector<char> host;
         WorkSet ws_all, ws_fresh;
-        size_t end = build(all, ws_all, host, 0);
-        if (!fresh.empty()) end = build(fresh, ws_fresh, host, end);
+        size_t end = build(all, ws_all, host, 0, MODE_UPDATE);
+        if (!fresh.empty()) end = build(fresh, ws_fresh, host, end, MODE_SUMS);
         ensure_meta(end);
         BNMF_HIP(hipMemcpyAsync(meta_.ptr, host.data(), end, hipMemcpyHostToDevice, stream_));
         char* m = meta_.as<char>();
@@ -437,7 +438,7 @@ class Server {
         if (!score.empty()) {
             WorkSet ws_sc;
             std::vector<char> h2;
-            const size_t e2 = build(score, ws_sc, h2, 0);
+            const size_t e2 = build(score, ws_sc, h2, 0, MODE_SCORE);
             // a second region after the step's lists, so they stay valid
             const size_t base = align_up(end);
             ensure_meta(base + e2);

@@ -123,6 +123,25 @@ bnmf_status bnmf_batch_get_info(bnmf_batch* b, bnmf_batch_info* info);
 void bnmf_batch_destroy(bnmf_batch* b);
 
 /* ------------------------------------------------------------------------
+ * Tile table (written by the autotuner, bnmf_hip.autotune): rows per pass
+ * tile for the shape class of an instance. z classes end at z = 4, 8, 12,
+ * 16, 24, 32, 48, 64; size classes split fibers = x*y at 2^14, 2^16, 2^18,
+ * 2^20. rows = 0 selects the built-in rule. The table is loaded on first use
+ * from $BNMF_TUNE_FILE, else from tune/gfx950.txt next to the library's
+ * directory. An instance is planned when its batch or request is created,
+ * so a table change affects later solves only. No counterpart in the
+ * reference (its kernels use fixed block shapes).
+ * ---------------------------------------------------------------------- */
+bnmf_status bnmf_tune_set(int dtype, size_t z, size_t fibers, unsigned rows);
+bnmf_status bnmf_tune_get(int dtype, size_t z, size_t fibers, unsigned* rows);
+bnmf_status bnmf_tune_load(const char* path);
+bnmf_status bnmf_tune_save(const char* path);
+/* Rows per tile an instance of this shape gets now (table or rule). */
+unsigned bnmf_tile_rows(int dtype, size_t x, size_t y, size_t z);
+/* Path of the loaded table file, "" if none (valid until the next load). */
+const char* bnmf_tune_source(void);
+
+/* ------------------------------------------------------------------------
  * Serving: a continuous-batching solve server. Requests are admitted in
  * arrival order while their HBM footprint (S plus the per-instance
  * workspace, bnmf_server_request_bytes) fits in the budget; a request larger

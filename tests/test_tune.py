@@ -1,0 +1,65 @@
+"""Tile table of the autotuner (host side, no device work): class lookup,
+the tile height an instance gets, and the text file round trip."""
+import numpy as np
+import pytest
+
+Z_REPS = (4, 8, 12, 16, 24, 32, 48, 64)
+F_REPS = (1 << 12, 1 << 15, 1 << 17, 1 << 19, 1 << 21)
+
+
+@pytest.fixture
+def table(hip):
+    """Snapshot of every class, restored afterwards (the table is global)."""
+    snap = {(dt, z, f): hip.tune_get(dt, z, f)
+            for dt in (np.float32, np.float64) for z in Z_REPS for f in F_REPS}
+    yield hip
+    for (dt, z, f), rows in snap.items():
+        hip.tune_set(dt, z, f, rows)
+
+
+def test_classes_and_rows(table):
+    hip = table
+    hip.tune_set(np.float32, 8, 1000 * 10, 0)
+    assert hip.tile_rows(np.float32, 1000, 10, 8) == 64       # built-in rule
+    other_z = hip.tune_get(np.float32, 9, 2000)
+    other_f = hip.tune_get(np.float32, 8, 1 << 14)
+    hip.tune_set(np.float32, 8, 1000 * 10, 16)
+    assert hip.tune_get(np.float32, 5, 2000) == 16           # same class: z 5..8, x*y < 2^14
+    assert hip.tune_get(np.float32, 9, 2000) == other_z      # next z class untouched
+    assert hip.tune_get(np.float32, 8, 1 << 14) == other_f   # next size class untouched
+    assert hip.tile_rows(np.float32, 1000, 10, 8) == 16
+    assert hip.tile_rows(np.float64, 1000, 10, 8) == 64       # per dtype
+    hip.tune_set(np.float32, 8, 1000 * 10, 128)
+    assert hip.tile_rows(np.float32, 1000, 10, 8) == 128
+    # rows are a multiple of 4 and cover x evenly
+    hip.tune_set(np.float32, 8, 1000 * 10, 48)
+    r = hip.tile_rows(np.float32, 1000, 10, 8)
+    assert r % 4 == 0 and r >= 48 and -(-1000 // r) == -(-1000 // 48)
+
+
+def test_invalid_entries(table):
+    hip = table
+    with pytest.raises(ValueError):
+        hip.tune_set(np.float32, 8, 100, 3)
+    with pytest.raises(ValueError):
+        hip.tune_set(np.float32, 65, 100, 64)
+    assert hip.tile_rows(np.float32, 10, 10, 65) == 0
+
+
+def test_save_load_roundtrip(table, tmp_path):
+    hip = table
+    hip.tune_set(np.float32, 30, 1 << 17, 32)
+    hip.tune_set(np.float64, 60, 1 << 21, 256)
+    path = tmp_path / "t.txt"
+    hip.tune_save(str(path))
+    text = path.read_text()
+    assert "f32 5 2 32" in text and "f64 7 4 256" in text
+    hip.tune_set(np.float32, 30, 1 << 17, 0)
+    hip.tune_load(str(path))
+    assert hip.tune_get(np.float32, 30, 1 << 17) == 32
+    assert hip.tune_source() == str(path)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("f32 9 0 64\n")
+    with pytest.raises(ValueError):
+        hip.tune_load(str(bad))
+    assert hip.tune_get(np.float32, 30, 1 << 17) == 32   # a bad file changes nothing
