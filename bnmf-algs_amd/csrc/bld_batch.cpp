@@ -125,6 +125,7 @@ InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z, bool wit
     L.off_psib = take((size_t)y * z * es);
     L.off_psia = take((size_t)x * z * es);
     L.off_partA = take((size_t)L.nA * z * es);
+    L.off_psiA = take((size_t)z * es);
     L.off_pjk = take((size_t)L.n_ti * y * z * 8);  // double capacity (scoring)
     L.off_ipk = take((size_t)L.n_tj * x * z * 8);
     L.off_score = take(((size_t)L.n_ti * L.n_tj + L.nA + L.n_cj + (size_t)L.nA * z) * 8);
@@ -145,6 +146,7 @@ BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double 
     d.psib = p + L.off_psib;
     d.psia = p + L.off_psia;
     d.partA = p + L.off_partA;
+    d.psiA = p + L.off_psiA;
     d.part_pjk = p + L.off_pjk;
     d.part_ipk = p + L.off_ipk;
     d.score = reinterpret_cast<double*>(p + L.off_score);
@@ -408,7 +410,7 @@ void Batch::enqueue_iteration(bool first) {
     const BldInst* d = reinterpret_cast<const BldInst*>(m + off_desc_);
     BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(n_prep_items_), stream_, d,
                          reinterpret_cast<const PrepItem*>(m + off_prep_items_),
-                         reinterpret_cast<const uint32_t*>(m + off_prep_count_)));
+                         reinterpret_cast<const uint32_t*>(m + off_prep_count_), (uint32_t)layouts_.size()));
     enqueue_pass(MODE_UPDATE, first);
 }
 
@@ -501,43 +503,63 @@ void Batch::time_pass(int reps, double* pass_ms, double* iter_ms) {
         *pass_ms = *iter_ms = 0.0;
         return;
     }
-    std::vector<hipEvent_t> ea(reps), eb(reps);
-    hipEvent_t e0, e1;
-    BNMF_HIP(hipEventCreate(&e0));
-    BNMF_HIP(hipEventCreate(&e1));
-    for (int r = 0; r < reps; ++r) {
-        BNMF_HIP(hipEventCreate(&ea[r]));
-        BNMF_HIP(hipEventCreate(&eb[r]));
+    // Two graphs, launched the way solves launch them: reps update passes
+    // (every pass launch of an iteration, forked over the side streams),
+    // and reps whole iterations (prep + pass). Each is replayed once to
+    // warm up, then timed with events on the batch stream.
+    auto capture = [&](bool with_prep, hipGraphExec_t* out) {
+        hipGraph_t g = nullptr;
+        BNMF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int r = 0; r < reps; ++r) {
+                if (with_prep)
+                    enqueue_iteration(false);
+                else
+                    enqueue_pass(MODE_UPDATE, false);
+            }
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        BNMF_HIP(hipStreamEndCapture(stream_, &g));
+        hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        BNMF_HIP(e);
+    };
+    hipGraphExec_t gp = nullptr, gi = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    auto cleanup = [&] {
+        if (gp) (void)hipGraphExecDestroy(gp);
+        if (gi) (void)hipGraphExecDestroy(gi);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (e2) (void)hipEventDestroy(e2);
+    };
+    try {
+        capture(false, &gp);
+        capture(true, &gi);
+        BNMF_HIP(hipEventCreate(&e0));
+        BNMF_HIP(hipEventCreate(&e1));
+        BNMF_HIP(hipEventCreate(&e2));
+        BNMF_HIP(hipGraphLaunch(gp, stream_));
+        BNMF_HIP(hipGraphLaunch(gi, stream_));
+        BNMF_HIP(hipEventRecord(e0, stream_));
+        BNMF_HIP(hipGraphLaunch(gp, stream_));
+        BNMF_HIP(hipEventRecord(e1, stream_));
+        BNMF_HIP(hipGraphLaunch(gi, stream_));
+        BNMF_HIP(hipEventRecord(e2, stream_));
+        BNMF_HIP(hipEventSynchronize(e2));
+        float tp = 0.f, ti = 0.f;
+        BNMF_HIP(hipEventElapsedTime(&tp, e0, e1));
+        BNMF_HIP(hipEventElapsedTime(&ti, e1, e2));
+        *pass_ms = (double)tp / reps;
+        *iter_ms = (double)ti / reps;
+    } catch (...) {
+        cleanup();
+        throw;
     }
-    char* m = meta_.as<char>();
-    const BldInst* d = reinterpret_cast<const BldInst*>(m + off_desc_);
-    BNMF_HIP(hipEventRecord(e0, stream_));
-    for (int r = 0; r < reps; ++r) {
-        BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(n_prep_items_), stream_, d,
-                             reinterpret_cast<const PrepItem*>(m + off_prep_items_),
-                             reinterpret_cast<const uint32_t*>(m + off_prep_count_)));
-        BNMF_HIP(hipEventRecord(ea[r], stream_));
-        enqueue_pass(MODE_UPDATE, false);
-        BNMF_HIP(hipEventRecord(eb[r], stream_));
-    }
-    BNMF_HIP(hipEventRecord(e1, stream_));
-    BNMF_HIP(hipEventSynchronize(e1));
-    double tot = 0.0;
-    for (int r = 0; r < reps; ++r) {
-        float ms = 0.f;
-        BNMF_HIP(hipEventElapsedTime(&ms, ea[r], eb[r]));
-        tot += ms;
-    }
-    float all = 0.f;
-    BNMF_HIP(hipEventElapsedTime(&all, e0, e1));
-    *pass_ms = tot / reps;
-    *iter_ms = (double)all / reps;
-    for (int r = 0; r < reps; ++r) {
-        (void)hipEventDestroy(ea[r]);
-        (void)hipEventDestroy(eb[r]);
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    cleanup();
 }
 
 void Batch::tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp) {
@@ -557,7 +579,7 @@ void Batch::tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp) {
                              reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
     BNMF_HIP(launch_prep(dtype_, psi_, true, dim3(n_prep_items_), stream_, dd.as<BldInst>(),
                          reinterpret_cast<const PrepItem*>(m + off_prep_items_),
-                         reinterpret_cast<const uint32_t*>(m + off_prep_count_)));
+                         reinterpret_cast<const uint32_t*>(m + off_prep_count_), (uint32_t)layouts_.size()));
     BNMF_HIP(hipStreamSynchronize(stream_));
     if (S_pjk) BNMF_HIP(hipMemcpy(S_pjk, arena_.as<char>(L.off_psib), (size_t)L.y * L.z * es_, hipMemcpyDeviceToHost));
     if (S_ipk) BNMF_HIP(hipMemcpy(S_ipk, arena_.as<char>(L.off_psia), (size_t)L.x * L.z * es_, hipMemcpyDeviceToHost));

@@ -87,14 +87,7 @@ __global__ void __launch_bounds__(64 * WAVES)
         if constexpr (MODE == PASS_UPDATE) {
             const BNMF_GLOBAL T* __restrict__ psib_g = gconst<T>(I->psib) + (size_t)j0 * z;
             for (uint32_t e = tid; e < n_el; e += NT) psib_l[e] = psib_g[e];
-            if (tid < z) {
-                // A_k = sum_i alpha_eph(i,k), the argument of psi in
-                // update_grad_minus (bld_mult_funcs.hpp:213-220)
-                const BNMF_GLOBAL T* __restrict__ pA = gconst<T>(I->partA);
-                SumAcc<T> A;
-                for (uint32_t c = 0; c < I->nA; ++c) A.add(pA[(size_t)c * z + tid]);
-                psiA_l[tid] = psi_sel<PSI, T>(A.get());
-            }
+            if (tid < z) psiA_l[tid] = gconst<T>(I->psiA)[tid];
             __syncthreads();
         }
 
@@ -440,6 +433,36 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// psi(A_k), A_k = sum_i alpha_eph(i,k) (the first term of update_grad_minus,
+// bld_mult_funcs.hpp:213-220), once per instance after prep: the prep row
+// chunks' column sums (partA) are reduced by P threads per k over strided
+// chunks, then the P partials are added in thread order (fixed tree).
+template <typename T, int PSI>
+__global__ void __launch_bounds__(256)
+    bld_psiA_kernel(const BldInst* __restrict__ insts, uint32_t n_inst) {
+    __shared__ T part[256];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t n = blockIdx.x; n < n_inst; n += gridDim.x) {
+        const BldInst* __restrict__ I = insts + n;
+        const uint32_t z = I->z, nA = I->nA;
+        if ((size_t)I->x * I->y == 0) continue;
+        const uint32_t P = 256u / z;  // partial sums per k (z <= 64: P >= 4)
+        const uint32_t k = tid % z, p = tid / z;
+        const BNMF_GLOBAL T* __restrict__ pA = gconst<T>(I->partA);
+        SumAcc<T> A;
+        if (p < P)
+            for (uint32_t c = p; c < nA; c += P) A.add(pA[(size_t)c * z + k]);
+        part[tid] = A.get();
+        __syncthreads();
+        if (tid < z) {
+            SumAcc<T> B;
+            for (uint32_t q = 0; q < P; ++q) B.add(part[q * z + tid]);
+            gmut<T>(I->psiA)[tid] = psi_sel<PSI, T>(B.get());
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // log-marginal scoring (alloc_model_funcs.hpp:574-586), FP64 throughout.
 // Needs a PASS_SCORE pass first: double partials of S_+jk / S_i+k and the
@@ -677,18 +700,24 @@ void read_layout(int dtype, uint32_t z, uint32_t* rw, uint32_t* hs, uint32_t* hm
 }
 
 hipError_t launch_prep(int dtype, int psi, bool raw, dim3 grid, hipStream_t st,
-                       const BldInst* insts, const PrepItem* items, const uint32_t* n_items) {
+                       const BldInst* insts, const PrepItem* items, const uint32_t* n_items,
+                       uint32_t n_inst) {
+    const dim3 gA(n_inst < 65535u ? (n_inst ? n_inst : 1u) : 65535u);
     if (dtype == 0) {
-        if (raw)
+        if (raw) {
             hipLaunchKernelGGL((bld_prep_kernel<float, PSI_F32, true>), grid, dim3(256), 0, st, insts, items, n_items);
-        else
+        } else {
             hipLaunchKernelGGL((bld_prep_kernel<float, PSI_F32, false>), grid, dim3(256), 0, st, insts, items, n_items);
+            hipLaunchKernelGGL((bld_psiA_kernel<float, PSI_F32>), gA, dim3(256), 0, st, insts, n_inst);
+        }
     } else if (raw) {
         hipLaunchKernelGGL((bld_prep_kernel<double, PSI_F64_EXACT, true>), grid, dim3(256), 0, st, insts, items, n_items);
     } else if (psi == PSI_F64_APPR) {
         hipLaunchKernelGGL((bld_prep_kernel<double, PSI_F64_APPR, false>), grid, dim3(256), 0, st, insts, items, n_items);
+        hipLaunchKernelGGL((bld_psiA_kernel<double, PSI_F64_APPR>), gA, dim3(256), 0, st, insts, n_inst);
     } else {
         hipLaunchKernelGGL((bld_prep_kernel<double, PSI_F64_EXACT, false>), grid, dim3(256), 0, st, insts, items, n_items);
+        hipLaunchKernelGGL((bld_psiA_kernel<double, PSI_F64_EXACT>), gA, dim3(256), 0, st, insts, n_inst);
     }
     return hipGetLastError();
 }

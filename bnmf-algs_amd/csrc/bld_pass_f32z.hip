@@ -150,14 +150,9 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
         {
             const BNMF_GLOBAL float* __restrict__ psib_g = gconst<float>(I->psib) + (size_t)j0 * Z;
             for (uint32_t e = tid; e < n_el; e += NT) psib_l[e] = psib_g[e];
-            if (tid < Z) {
-                // A_k = sum_i alpha_eph(i,k), the argument of psi in
-                // update_grad_minus (bld_mult_funcs.hpp:213-220)
-                const BNMF_GLOBAL float* __restrict__ pA = gconst<float>(I->partA);
-                SumAcc<float> A;
-                for (uint32_t c = 0; c < I->nA; ++c) A.add(pA[(size_t)c * Z + tid]);
-                psiA_l[tid] = psi_sel<PSI_F32, float>(A.get());
-            }
+            // psi(sum_i alpha_eph(i,k)) (update_grad_minus,
+            // bld_mult_funcs.hpp:213-220), from the prep launch
+            if (tid < Z) psiA_l[tid] = gconst<float>(I->psiA)[tid];
             __syncthreads();
         }
 

@@ -10,7 +10,8 @@
 //   alpha    double[x], beta double[z]    model parameters (Params::alpha/beta)
 //   psib     T[y*z]          psi(S_+jk + beta_k)          (beta_eph, :152-163)
 //   psia     T[x*z]          psi(S_i+k + alpha_i)         (alpha_eph, :130-141)
-//   partA    ACC[nA*z]       column sums of alpha_eph over row chunks of 64
+//   partA    ACC[nA*z]       column sums of alpha_eph over row chunks of 16
+//   psiA     T[z]            psi(sum_i alpha_eph(i,k))    (update_grad_minus)
 //   part_pjk ACC[n_ti*y*z]   per-row-tile partial S_+jk
 //   part_ipk ACC[n_tj*x*z]   per-column-tile partial S_i+k
 //   score    double[n_ti*n_tj + nA + n_cj]  log-marginal partials
@@ -23,9 +24,10 @@ namespace bnmf {
 // Fibers (columns j) per tile: one wavefront lane per fiber.
 constexpr uint32_t kTileCols = 64;
 // Rows per prep chunk (alpha_eph column-sum granularity) and columns per
-// prep chunk.
-constexpr uint32_t kPrepRows = 64;
-constexpr uint32_t kPrepCols = 64;
+// prep chunk: small, so that even one large instance gives the prep launch
+// hundreds of blocks.
+constexpr uint32_t kPrepRows = 16;
+constexpr uint32_t kPrepCols = 16;
 
 struct BldInst {
     void* S;              // updated tensor (pass output)
@@ -36,6 +38,7 @@ struct BldInst {
     void* psib;
     void* psia;
     void* partA;
+    void* psiA;           // psi(sum_i alpha_eph(i,k)), z values
     void* part_pjk;
     void* part_ipk;
     double* score;        // score partials (log-marginal), may be null

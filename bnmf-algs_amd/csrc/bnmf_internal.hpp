@@ -44,8 +44,11 @@ hipError_t launch_pass(int dtype, int psi, int mode, int zb, int rw, dim3 grid, 
                        const BldInst* insts, const PassItem* items, const uint32_t* n_items,
                        uint32_t zmax);
 void read_layout(int dtype, uint32_t z, uint32_t* rw, uint32_t* hs, uint32_t* hm);
+// raw = false: also the per-instance psi(sum_i alpha_eph) kernel over n_inst
+// descriptors
 hipError_t launch_prep(int dtype, int psi, bool raw, dim3 grid, hipStream_t st,
-                       const BldInst* insts, const PrepItem* items, const uint32_t* n_items);
+                       const BldInst* insts, const PrepItem* items, const uint32_t* n_items,
+                       uint32_t n_inst);
 hipError_t launch_score(dim3 grid, hipStream_t st, const BldInst* insts, const PrepItem* items,
                         const uint32_t* n_items, uint32_t n_inst);
 hipError_t launch_psi_eval(int kind, const double* xin, double* out, size_t n, hipStream_t st);
@@ -116,7 +119,7 @@ struct InstSpec {
 // Byte layout of one instance inside an arena.
 struct InstLayout {
     uint32_t x, y, z, zb, rw, hs, hm, ti_rows, n_ti, n_tj, nA, n_cj;
-    size_t off_S, off_S0, off_X, off_alpha, off_beta, off_psib, off_psia, off_partA, off_pjk,
+    size_t off_S, off_S0, off_X, off_alpha, off_beta, off_psib, off_psia, off_partA, off_psiA, off_pjk,
         off_ipk, off_score, off_result;
     size_t bytes;  // total, 256-byte aligned
     size_t elements() const { return (size_t)x * y * z; }

@@ -421,7 +421,7 @@ class Server {
                 BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(ws_all.n_prep), stream_,
                                      reinterpret_cast<const BldInst*>(m + ws_all.off_desc),
                                      reinterpret_cast<const PrepItem*>(m + ws_all.off_prep),
-                                     reinterpret_cast<const uint32_t*>(m + ws_all.off_prep_count)));
+                                     reinterpret_cast<const uint32_t*>(m + ws_all.off_prep_count), ws_all.n_inst));
             passes(ws_all, MODE_UPDATE);
         }
         // requests that finish in this step, scored if asked

@@ -105,9 +105,11 @@ bnmf_status bnmf_batch_synchronize(bnmf_batch* b);
 bnmf_status bnmf_batch_get_S(bnmf_batch* b, size_t idx, void* S_out);
 /* FP64 log_marginal_S of every instance's current S; out has n entries. */
 bnmf_status bnmf_batch_log_marginal(bnmf_batch* b, double* out);
-/* Device-side timing: mean milliseconds of one fused pass launch (all
- * buckets) and of one whole iteration (prep + pass), over `reps`
- * iterations measured with HIP events on the batch stream. */
+/* Device-side timing: mean milliseconds of one update pass (all its kernel
+ * launches) and of one whole iteration (prep + pass), each from a captured
+ * graph of `reps` repetitions replayed as solves replay theirs and timed
+ * with HIP events on the batch stream. Advances S by 4*reps passes; solve
+ * again (from S0) to restore a result. */
 bnmf_status bnmf_batch_time_pass(bnmf_batch* b, int reps, double* pass_ms, double* iter_ms);
 
 typedef struct bnmf_batch_info {
