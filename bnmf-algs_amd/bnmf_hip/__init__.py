@@ -28,7 +28,9 @@ from typing import Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbnmf_hip.so")
+# BNMF_HIP_LIB selects another build of the library (e.g. for comparisons)
+LIB_PATH = os.environ.get("BNMF_HIP_LIB") or os.path.join(os.path.dirname(_HERE), "lib",
+                                                           "libbnmf_hip.so")
 
 F32, F64 = 0, 1
 OK, ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE = 0, 1, 2, 3

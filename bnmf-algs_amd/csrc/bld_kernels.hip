@@ -373,6 +373,22 @@ __global__ void __launch_bounds__(64 * WAVES)
 // ---------------------------------------------------------------------------
 // prep kernel
 // ---------------------------------------------------------------------------
+// s += base[t * stride] for t = 0..n-1 in order; the loads are issued eight
+// at a time ahead of their (sequential) additions.
+template <typename T>
+__device__ __forceinline__ void add_strided(SumAcc<T>& s, const T* __restrict__ base,
+                                            size_t stride, uint32_t n) {
+    uint32_t t = 0;
+    for (; t + 8 <= n; t += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = base[(size_t)(t + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s.add(v[u]);
+    }
+    for (; t < n; ++t) s.add(base[(size_t)t * stride]);
+}
+
 template <typename T, int PSI, bool RAW>
 __global__ void __launch_bounds__(256)
     bld_prep_kernel(const BldInst* __restrict__ insts, const PrepItem* __restrict__ items,
@@ -395,7 +411,7 @@ __global__ void __launch_bounds__(256)
                 const size_t idx = (size_t)c0 * z + e;
                 const uint32_t k = e - fiber_of(e, inv_z) * z;
                 SumAcc<T> s;
-                for (uint32_t t = 0; t < I->n_ti; ++t) s.add(pp[t * stride + idx]);
+                add_strided(s, pp + idx, stride, I->n_ti);
                 if constexpr (RAW) {
                     psib[idx] = s.get();
                 } else {
@@ -413,7 +429,7 @@ __global__ void __launch_bounds__(256)
                 const size_t idx = (size_t)r0 * z + e;
                 const uint32_t r = fiber_of(e, inv_z);
                 SumAcc<T> s;
-                for (uint32_t t = 0; t < I->n_tj; ++t) s.add(pp[t * stride + idx]);
+                add_strided(s, pp + idx, stride, I->n_tj);
                 if constexpr (RAW) {
                     psia[idx] = s.get();
                 } else {
@@ -450,8 +466,8 @@ __global__ void __launch_bounds__(256)
         const uint32_t k = tid % z, p = tid / z;
         const BNMF_GLOBAL T* __restrict__ pA = gconst<T>(I->partA);
         SumAcc<T> A;
-        if (p < P)
-            for (uint32_t c = p; c < nA; c += P) A.add(pA[(size_t)c * z + k]);
+        if (p < P && p < nA)
+            add_strided(A, (const T*)(pA + (size_t)p * z + k), (size_t)P * z, (nA - p + P - 1) / P);
         part[tid] = A.get();
         __syncthreads();
         if (tid < z) {
@@ -673,8 +689,8 @@ hipError_t launch_pass(int dtype, int psi, int mode, int zb, int rw, dim3 grid, 
 }
 
 // LDS read width of an instance of depth z (update pass) and the XOR mask of
-// its chunk visit order. A read of RW elements serves 128 / (RW*sizeof(T))
-// lanes per LDS cycle; fibers of lanes l and l' start on the same bank group
+// its chunk visit order. A read of RW elements serves
+// kLdsBankBytes / (RW*sizeof(T)) lanes per LDS cycle; fibers of lanes l and l' start on the same bank group
 // when l*z and l'*z agree modulo the group count, so lanes that repeat a
 // group (period lanes_per_cycle / P) get distinct XOR masks below P, where P
 // is the largest power of two dividing the chunk count (so m ^ mask stays a
@@ -686,7 +702,7 @@ void read_layout(int dtype, uint32_t z, uint32_t* rw, uint32_t* hs, uint32_t* hm
     else
         r = (z % 2 == 0) ? 2 : 1;
     const uint32_t es = dtype == 0 ? 4 : 8;
-    const uint32_t lanes_per_cycle = 128u / (r * es);
+    const uint32_t lanes_per_cycle = kLdsBankBytes / (r * es);
     const uint32_t C = z / r;
     uint32_t P = 1;
     while (P < lanes_per_cycle && (C % (P * 2)) == 0) P *= 2;

@@ -28,13 +28,19 @@
 
 namespace bnmf {
 
+// Buffer resource word 3 for raw (untyped) buffer access on gfx9-family
+// GPUs, and an offset past any resource of this kernel: a store given it is
+// dropped by the range check.
+constexpr int kBufWord3 = 0x00020000;
+constexpr uint32_t kBufDrop = 0x80000000u;
+
 // Compile-time form of read_layout() for float: LDS read width RW and the XOR
 // mask of the chunk visit order (lane l visits chunk m as m ^ ((l >> HS) & HM)).
 template <int Z>
 struct F32Layout {
     static constexpr int RW = (Z % 4 == 0) ? 4 : ((Z % 2 == 0) ? 2 : 1);
     static constexpr int NCH = Z / RW;
-    static constexpr int LPC = 128 / (RW * 4);
+    static constexpr int LPC = (int)kLdsBankBytes / (RW * 4);
     static constexpr int p_of() {
         int p = 1;
         if (RW == 1) return 1;
@@ -83,13 +89,13 @@ constexpr int f32z_waves() {
 
 // Waves per SIMD the register allocation must leave room for.
 #ifndef BNMF_F32Z_WPE
-#define BNMF_F32Z_WPE(Z) ((Z) <= 10 ? 4 : (Z) <= 18 ? 3 : (Z) <= 31 ? 2 : 1)
+#define BNMF_F32Z_WPE(Z) ((Z) <= 12 ? 4 : (Z) <= 20 ? 3 : (Z) <= 38 ? 2 : 1)
 #endif
 
 // Memory pipelining: a wave issues the next row's loads (S chunks, X, psia)
 // right after moving the current row into LDS, and every global store of a
-// row is unconditional (lanes with nothing to write target the 64-byte
-// slack after S). The instruction count between a row's loads and their
+// row is an unconditional buffer store (lanes with nothing to write pass an
+// out-of-range offset, which the hardware drops). The instruction count between a row's loads and their
 // first use is then fixed, so the wait for them leaves the previous row's
 // stores in flight instead of draining the whole memory queue.
 template <int Z, int WAVES>
@@ -108,6 +114,7 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     constexpr uint32_t ZQ = Lay::ZQ;
     constexpr uint32_t L = 64u / ZQ;
     typedef float V16 __attribute__((ext_vector_type(4)));
+    typedef uint32_t U16 __attribute__((ext_vector_type(4)));
     typedef typename VecR<float, RW>::type VR;
 
     const uint32_t n_items = *n_items_p;
@@ -136,11 +143,6 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 
         const BNMF_GLOBAL float* __restrict__ Ssrc = gconst<float>(I->S_src);
         BNMF_GLOBAL float* __restrict__ Sdst = gmut<float>(I->S);
-        // in the 64 bytes after S (16-byte aligned): target of the stores
-        // that have nothing to write
-        BNMF_GLOBAL float* __restrict__ trash =
-            (BNMF_GLOBAL float*)(((uintptr_t)(Sdst + (size_t)x * y * Z) + 15u) & ~(uintptr_t)15u) +
-            (lane & 1u) * 4u;
         const BNMF_GLOBAL float* __restrict__ Xg = gconst<float>(I->X);
         const BNMF_GLOBAL float* __restrict__ psia_g = gconst<float>(I->psia);
         const float epsf = static_cast<float>(I->eps);
@@ -288,27 +290,32 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                     gmut<float>(I->part_ipk)[((size_t)item.tj * x + i) * Z + kq] = rs;
             }
 
-            // the row back to S: 16-byte chunks that lie inside the row
-            // segment go to their place, the rest to the slack; then the (at
-            // most two) chunks straddling the segment ends element by
-            // element, lanes 0-3 the head chunk and lanes 4-7 the tail chunk
+            // the row back to S through a buffer resource spanning the row's
+            // chunks: every lane issues its stores, and a lane with nothing
+            // to write gives an offset past the resource, which the hardware
+            // drops. 16-byte chunks that lie inside the row segment go whole;
+            // the (at most two) chunks straddling the segment ends go element
+            // by element, lanes 0-3 the head chunk and lanes 4-7 the tail.
             {
-                BNMF_GLOBAL V16* __restrict__ dst = (BNMF_GLOBAL V16*)(Sdst + (cur_s0 - cur_dl));
+                const __amdgpu_buffer_rsrc_t dst = __builtin_amdgcn_make_buffer_rsrc(
+                    (void*)(Sdst + (cur_s0 - cur_dl)), (short)0, (int)(cur_nch * 16u), kBufWord3);
                 const uint32_t lo = cur_dl, hi = cur_dl + n_el;
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     const uint32_t q = (uint32_t)c * 64u + lane;
                     const uint32_t e0 = q * VEC;
                     const bool inside = q < cur_nch && e0 >= lo && e0 + VEC <= hi;
-                    BNMF_GLOBAL V16* to = inside ? dst + q : (BNMF_GLOBAL V16*)trash;
-                    *to = reinterpret_cast<const V16*>(raw)[q];
+                    const V16 v = reinterpret_cast<const V16*>(raw)[q];
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        __builtin_bit_cast(U16, v), dst, (int)(inside ? q * 16u : kBufDrop), 0, 0);
                 }
                 const uint32_t qe = lane < 4u ? 0u : cur_nch - 1u;
                 const uint32_t e = qe * VEC + (lane & 3u);
                 const bool mine = lane < 8u && e >= lo && e < hi &&
                                   ((qe * VEC < lo) || (qe * VEC + VEC > hi));
-                BNMF_GLOBAL float* to = mine ? (BNMF_GLOBAL float*)dst + e : trash;
-                *to = raw[min(e, (uint32_t)Lds::raw_stride / 4u - 1u)];
+                const float v = raw[min(e, (uint32_t)Lds::raw_stride / 4u - 1u)];
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), dst,
+                                                      (int)(mine ? e * 4u : kBufDrop), 0, 0);
             }
             wave_sync();
         };

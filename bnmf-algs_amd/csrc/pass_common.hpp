@@ -137,6 +137,9 @@ struct PassLds {
 };
 
 constexpr size_t kLdsBytes = 160 * 1024;
+// Bytes of LDS bank width the chunk visit order spreads a cycle over: lanes
+// whose addresses agree modulo this in one access cycle conflict.
+constexpr uint32_t kLdsBankBytes = 128;
 
 // Wavefronts per pass block: four (one per SIMD), so that two blocks can
 // share a CU and hide each other's tile prologue and epilogue; fewer when
