@@ -526,53 +526,71 @@ def tile_rows(dtype, x: int, y: int, z: int) -> int:
     return int(lib().bnmf_tile_rows(_code(_np_dtype(dtype)), int(x), int(y), int(z)))
 
 
-def autotune(dtype=np.float32, z_classes=TUNE_Z_CLASSES, fiber_classes=TUNE_FIBER_CLASSES,
-             candidates=(16, 32, 64, 128, 256), reps: int = 10, trials: int = 3,
-             target_elements: int = 1 << 26,
-             max_instances: int = 256, path: Optional[str] = None, seed: int = 0x7A7E,
-             log=None) -> dict:
-    """Times the fused pass for each tile height candidate on one
-    representative workload per shape class and keeps the fastest in the
-    tile table (saved to `path` if given).
+def _tune_class(z: int, fibers: int):
+    zc = next(c for c, hi in enumerate(TUNE_Z_CLASSES) if z <= hi or c == len(TUNE_Z_CLASSES) - 1)
+    fc = next(c for c, (lo, hi) in enumerate(TUNE_FIBER_CLASSES)
+              if fibers < hi or c == len(TUNE_FIBER_CLASSES) - 1)
+    return zc, fc
 
-    A class's workload is n instances of x = y = sqrt(geometric mid of the
-    fiber range) and z = the class's largest depth, with n chosen so the
-    batch holds about `target_elements` tensor elements (small instances
-    are solved batched). X comes from the allocation model (a = 40, b = 1).
-    Each candidate's time is the best of `trials` timings of `reps` passes
-    (graph-launched, as solves run them). Candidates that give the same
-    tiling as an earlier one are skipped.
-    Returns {(zc, fc): {"rows": best, "pass_ms": {rows: ms}}}."""
+
+def autotune(dtype=np.float32, instances: Optional[Sequence[Instance]] = None,
+             z_classes=TUNE_Z_CLASSES, fiber_classes=TUNE_FIBER_CLASSES,
+             candidates=(16, 32, 64, 128, 256), reps: int = 10, trials: int = 3,
+             target_elements: int = 1 << 26, max_instances: int = 256,
+             path: Optional[str] = None, seed: int = 0x7A7E, use_psi_appr: bool = True,
+             log=None) -> dict:
+    """Times the fused pass for each tile height candidate, per shape class,
+    and keeps the fastest in the tile table (saved to `path` if given).
+
+    With `instances` (a workload), each class present in it is timed on its
+    own instances, solved together as a batch. Without, each class of
+    `z_classes` x `fiber_classes` gets a synthetic workload: n instances of
+    x = y = sqrt(geometric mid of the fiber range) and z = the class's
+    largest depth, n chosen so the batch holds about `target_elements`
+    elements, X from the allocation model (a = 40, b = 1).
+    A candidate's time is the best of `trials` timings of `reps` passes
+    (graph-launched, as solves run them); candidates that tile every
+    instance of the class as an earlier one did are skipped.
+    Returns {(zc, fc): {"rows": best, "pass_ms": {rows: ms}, ...}}."""
     dt = _np_dtype(dtype)
-    rng = np.random.default_rng(seed)
+    groups = {}
+    if instances is not None:
+        for inst in instances:
+            x, y = np.shape(inst.X)
+            if x * y * inst.z:
+                groups.setdefault(_tune_class(inst.z, x * y), []).append(inst)
+    else:
+        rng = np.random.default_rng(seed)
+        for z in z_classes:
+            for lo, hi in fiber_classes:
+                side = max(1, int(round(np.sqrt(np.sqrt(lo * (hi - 1))))))
+                n = int(min(max_instances, max(1, target_elements // (side * side * z))))
+                p = Params(40.0, 1.0, [1.0] * side, [1.0] * z)
+                groups[_tune_class(z, side * side)] = [
+                    Instance(X=sample_X(int(rng.integers(1 << 62)), (side, side, z), p).astype(dt),
+                             z=z, params=p, seed=int(rng.integers(1 << 62))) for _ in range(n)]
     results = {}
-    for zc, z in enumerate(z_classes):
-        for fc, (lo, hi) in enumerate(fiber_classes):
-            fib = int(np.sqrt(lo * (hi - 1)))
-            side = max(1, int(round(np.sqrt(fib))))
-            n = int(min(max_instances, max(1, target_elements // (side * side * z))))
-            p = Params(40.0, 1.0, [1.0] * side, [1.0] * z)
-            insts = []
-            for t in range(n):
-                X = sample_X(int(rng.integers(1 << 62)), (side, side, z), p).astype(dt)
-                insts.append(Instance(X=X, z=z, params=p, seed=int(rng.integers(1 << 62))))
-            timed, seen = {}, {}
-            for cand in candidates:
-                tune_set(dt, z, side * side, cand)
-                rows = tile_rows(dt, side, side, z)
-                if rows in seen:
-                    continue
-                seen[rows] = cand
-                with Batch(insts, dtype=dt, use_psi_appr=True) as b:
-                    b.solve(2)
-                    timed[cand] = min(b.time_pass(reps)[0] for _ in range(trials))
-            best = min(timed, key=timed.get)
-            tune_set(dt, z, side * side, best)
-            results[(zc, fc)] = {"rows": best, "pass_ms": timed, "shape": (side, side, z),
-                                 "instances": n}
-            if log:
-                log(f"[autotune] z<={z} fibers~{side * side} x{n}: " +
-                    " ".join(f"{c}:{t * 1e3:.0f}us" for c, t in timed.items()) + f" -> {best}")
+    for key in sorted(groups):
+        insts = groups[key]
+        shapes = [(np.shape(i.X)[0], np.shape(i.X)[1], i.z) for i in insts]
+        z0, f0 = shapes[0][2], shapes[0][0] * shapes[0][1]
+        timed, seen = {}, set()
+        for cand in candidates:
+            tune_set(dt, z0, f0, cand)
+            tiling = tuple(tile_rows(dt, *sh) for sh in shapes)
+            if tiling in seen:
+                continue
+            seen.add(tiling)
+            with Batch(insts, dtype=dt, use_psi_appr=use_psi_appr) as b:
+                b.solve(2)
+                timed[cand] = min(b.time_pass(reps)[0] for _ in range(trials))
+        best = min(timed, key=timed.get)
+        tune_set(dt, z0, f0, best)
+        results[key] = {"rows": best, "pass_ms": timed, "shape": shapes[0], "instances": len(insts)}
+        if log:
+            log(f"[autotune] class z{key[0]} f{key[1]} ({len(insts)} instances, e.g. "
+                f"{shapes[0]}): " + " ".join(f"{c}:{t * 1e3:.0f}us" for c, t in timed.items()) +
+                f" -> {best}")
     if path:
         tune_save(path)
     return results
