@@ -239,6 +239,9 @@ def main():
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="plan the headline batch with the built-in tile rule")
+    ap.add_argument("--tune-file", default=None,
+                    help="load this tile table instead of autotuning (e.g. for profiling runs)")
+    ap.add_argument("--save-tune", default=None, help="save the autotuned tile table here")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -269,7 +272,10 @@ def main():
     fibers3 = sum(i.X.shape[0] * i.X.shape[1] for i in c3)
     c3f = [as_dtype(i, hip, np.float32) for i in c3]
     tuned = None
-    if not args.no_autotune:
+    if args.tune_file:
+        hip.tune_load(args.tune_file)
+        tuned = {"table": args.tune_file}
+    elif not args.no_autotune:
         # tile heights per shape class, chosen by the autotuner on this
         # workload (untimed setup; the table applies to batches created after)
         log("[bench] autotuning tile heights on the batch's shape classes")
@@ -278,6 +284,8 @@ def main():
                            trials=2, log=log)
         tuned = {"seconds": round(time.perf_counter() - t0, 1), "classes": len(res),
                  "rows": {f"z{k[0]}f{k[1]}": v["rows"] for k, v in sorted(res.items())}}
+        if args.save_tune:
+            hip.tune_save(args.save_tune)
     b32 = hip.Batch(c3f, dtype=np.float32, use_psi_appr=True)
     log("[bench] timing batched FP32 solves")
     t_step = timed_solves(b32, iters3, args.steps, args.warmup)

@@ -310,7 +310,8 @@ def init_S(seed: int, X, z: int, dtype=np.float64) -> np.ndarray:
 
 def psi_eval(kind: int, x) -> np.ndarray:
     """Device digamma forms (0: FP32 bulk, 1: exact FP64, 2: psi_appr FP64,
-    3: FP32 psi(s+1) hot-path form evaluated at x = s+1)."""
+    3: FP32 psi(s+1) of the generic update pass at x = s+1, 4: FP32 psi of the
+    exact-z update pass, x >= 1)."""
     x = np.ascontiguousarray(x, dtype=np.float64).ravel()
     out = np.empty_like(x)
     _check(lib().bnmf_psi_eval(int(kind), _dptr(x), _dptr(out), x.size), "psi_eval")

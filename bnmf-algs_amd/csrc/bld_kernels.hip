@@ -750,6 +750,8 @@ __global__ void __launch_bounds__(256)
         r = (double)psi_f32(f);
     } else if (kind == 3) {
         r = (double)psi_sp1<PSI_F32, float>((float)v - 1.0f);
+    } else if (kind == 4) {
+        r = (double)psi_ge1_log2_f32((float)v) * 0.6931471805599453;
     } else if (kind == 1) {
         r = psi_f64(v);
     } else {

@@ -32,6 +32,7 @@ namespace bnmf {
 // GPUs, and an offset past any resource of this kernel: a store given it is
 // dropped by the range check.
 constexpr int kBufWord3 = 0x00020000;
+constexpr float kInvLn2 = 1.4426950408889634f;
 constexpr uint32_t kBufDrop = 0x80000000u;
 
 // Compile-time form of read_layout() for float: LDS read width RW and the XOR
@@ -89,7 +90,7 @@ constexpr int f32z_waves() {
 
 // Waves per SIMD the register allocation must leave room for.
 #ifndef BNMF_F32Z_WPE
-#define BNMF_F32Z_WPE(Z) ((Z) <= 12 ? 4 : (Z) <= 20 ? 3 : (Z) <= 38 ? 2 : 1)
+#define BNMF_F32Z_WPE(Z) ((Z) <= 12 ? 4 : (Z) <= 20 ? 3 : (Z) <= 37 ? 2 : 1)
 #endif
 
 // Memory pipelining: a wave issues the next row's loads (S chunks, X, psia)
@@ -149,22 +150,6 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
         const uint32_t lane_x = min(lane, nj - 1u);  // clamped fiber of this lane
         const uint32_t lane_z = min(lane, (uint32_t)Z - 1u);
 
-        {
-            const BNMF_GLOBAL float* __restrict__ psib_g = gconst<float>(I->psib) + (size_t)j0 * Z;
-            for (uint32_t e = tid; e < n_el; e += NT) psib_l[e] = psib_g[e];
-            // psi(sum_i alpha_eph(i,k)) (update_grad_minus,
-            // bld_mult_funcs.hpp:213-220), from the prep launch
-            if (tid < Z) psiA_l[tid] = gconst<float>(I->psiA)[tid];
-            __syncthreads();
-        }
-
-        float acc[Z];
-#pragma unroll
-        for (int k = 0; k < Z; ++k) acc[k] = 0.0f;
-
-        const uint32_t kq = lane & (ZQ - 1u);
-        const uint32_t part = lane >> Lay::ZQL;
-
         // prefetch registers of one row: S chunks from the aligned start
         // (chunk indices clamped to the row's last chunk), X_ij of the
         // lane's fiber and psi(alpha_eph)(i, lane)
@@ -187,13 +172,41 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
             d_out = d;
             n_out = n;
         };
+        // the tile's first row is in flight while psi(beta_eph) is staged
+        if (i < i1) issue_row(i, true, dl, nch);
+
+        {
+            const BNMF_GLOBAL float* __restrict__ psib_g = gconst<float>(I->psib) + (size_t)j0 * Z;
+            // psi values enter the pass in log2 units (psi / ln 2): the
+            // update is a ratio of psi differences, so the scale cancels,
+            // and psi(S+1) then needs no final multiply by ln 2
+            constexpr uint32_t PB = (64u * Z + NT - 1) / NT;  // elements per thread
+            float pbv[PB];
+#pragma unroll
+            for (uint32_t u = 0; u < PB; ++u) pbv[u] = psib_g[min(tid + u * NT, n_el - 1u)];
+#pragma unroll
+            for (uint32_t u = 0; u < PB; ++u)
+                if (tid + u * NT < n_el) psib_l[tid + u * NT] = pbv[u] * kInvLn2;
+            // psi(sum_i alpha_eph(i,k)) (update_grad_minus,
+            // bld_mult_funcs.hpp:213-220), from the prep launch
+            if (tid < Z) psiA_l[tid] = gconst<float>(I->psiA)[tid];
+            __syncthreads();
+        }
+
+        float acc[Z];
+#pragma unroll
+        for (int k = 0; k < Z; ++k) acc[k] = 0.0f;
+
+        const uint32_t kq = lane & (ZQ - 1u);
+        const uint32_t part = lane >> Lay::ZQL;
+
         // One row of the tile. The first row is peeled off the loop so that
         // at the loop head the previous row's stores always follow the
         // current row's loads in issue order.
         auto row_step = [&](uint32_t i) {
 #pragma unroll
             for (int c = 0; c < CPL; ++c) reinterpret_cast<V16*>(raw)[(uint32_t)c * 64u + lane] = pf[c];
-            if (lane < (uint32_t)Z) gm_l[lane] = psiA_l[lane] - pf_pa;
+            if (lane < (uint32_t)Z) gm_l[lane] = (psiA_l[lane] - pf_pa) * kInvLn2;
             // X_ij goes through LDS too: the prefetch register is then free
             // before the next row's load, and the loop carries no copy that
             // would have to wait for that load
@@ -211,37 +224,42 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                 float gp[Z];
                 float sv_keep[KEEP ? Z : 1];
                 float sumS = 0.0f, nom = 0.0f, den = 0.0f;
-                // eps = 1e-50 flushes to zero in float; an X_ij = 0 or
-                // all-zero fiber updates to exactly 0 as in the reference's
-                // double-promoted guard.
-                const bool live = xv > 0.0f;
-                const float xr = live ? 1.0f / (xv + epsf) : 0.0f;
-                auto phase_a = [&](auto psi_of) {
+                // Phase A: per element g = psi(beta_eph) - psi(S+1) (in log2
+                // units) and the fiber sums S_ij+, sum_k S*gm and sum_k S*g.
+                // S >= 0 throughout (S0 >= 0 and beta >= 1), so S + 1 >= 1
+                // and psi takes its x >= 1 form directly.
 #pragma unroll
-                    for (int m = 0; m < NCH; ++m) {
-                        const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
-                        const VR sv = *reinterpret_cast<const VR*>(fb + mm);
-                        const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
-                        const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+                for (int m = 0; m < NCH; ++m) {
+                    const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                    const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                    const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                    const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
 #pragma unroll
-                        for (int v = 0; v < RW; ++v) {
-                            const float sk = vget<float, RW>(sv, v);
-                            if constexpr (KEEP) sv_keep[m * RW + v] = sk;
-                            const float g = vget<float, RW>(pb, v) - psi_of(sk);
-                            gp[m * RW + v] = g;
-                            sumS += sk;
-                            const float t = sk * xr;
-                            nom = fmaf(t, vget<float, RW>(gv, v), nom);
-                            den = fmaf(t, g, den);
-                        }
+                    for (int v = 0; v < RW; ++v) {
+                        const float sk = vget<float, RW>(sv, v);
+                        if constexpr (KEEP) sv_keep[m * RW + v] = sk;
+                        const float g = vget<float, RW>(pb, v) - psi_ge1_log2_f32(sk + 1.0f);
+                        gp[m * RW + v] = g;
+                        sumS += sk;
+                        nom = fmaf(sk, vget<float, RW>(gv, v), nom);
+                        den = fmaf(sk, g, den);
                     }
-                };
-                // S >= 0 throughout (the instance has S0 >= 0 and beta >= 1),
-                // so s + 1 >= 1 and psi takes its x >= 1 form directly
-                phase_a([](float sk) { return psi_ge1_f32(sk + 1.0f); });
-                const bool upd = live && sumS != 0.0f;
-                const float x_over_s = upd ? xv / (sumS + epsf) : 0.0f;
-                const float den_e = den + epsf;
+                }
+                // Per-fiber constants. With Xh = 1/(X + eps) (X_reciprocal)
+                // the reference's nom and denom are Xh * sum, and its update
+                //   S *= (g + nom) * X / (S_ij+ + eps) / (gm + den + eps)
+                // is S *= (g + nom) / (gm * q + (den + eps) * q) with
+                // q = (S_ij+ + eps) / X. eps = 1e-50 flushes to zero in
+                // float; the reference's double-promoted guard makes an
+                // X_ij = 0 or all-zero fiber update to exactly 0, taken here
+                // as q = 0, (den + eps) * q = inf: the factor is then 0.
+                const bool upd = xv > 0.0f && sumS != 0.0f;
+                const float xh = upd ? 1.0f / (xv + epsf) : 0.0f;
+                nom *= xh;
+                den *= xh;
+                const float q = upd ? (sumS + epsf) / xv : 0.0f;
+                const float dq = upd ? (den + epsf) * q : __builtin_inff();
+                // Phase B: the multiplicative update and the S_+jk partials
 #pragma unroll
                 for (int m = 0; m < NCH; ++m) {
                     const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
@@ -256,9 +274,8 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
                     for (int v = 0; v < RW; ++v) {
                         const float sk = vget<float, RW>(sv, v);
-                        const float fct = (gp[m * RW + v] + nom) * x_over_s *
-                                          recip_fast<float>(vget<float, RW>(gv, v) + den_e);
-                        const float sn = upd ? sk * fct : 0.0f;
+                        const float r = recip_fast<float>(fmaf(vget<float, RW>(gv, v), q, dq));
+                        const float sn = sk * ((gp[m * RW + v] + nom) * r);
                         vset<float, RW>(sv, v, sn);
                         acc[m * RW + v] += sn;
                     }
@@ -320,7 +337,6 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
             wave_sync();
         };
         if (i < i1) {
-            issue_row(i, true, dl, nch);
             row_step(i);
             for (i += WAVES; i < i1; i += WAVES) row_step(i);
         }

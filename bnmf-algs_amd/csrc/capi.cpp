@@ -294,7 +294,7 @@ bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_
 
 bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n) {
     return guarded([&] {
-        BNMF_REQUIRE(kind >= 0 && kind <= 3, "kind must be 0..3");
+        BNMF_REQUIRE(kind >= 0 && kind <= 4, "kind must be 0..4");
         BNMF_REQUIRE(n == 0 || (x && out), "null argument");
         if (n == 0) return;
         bnmf::ensure_runtime();

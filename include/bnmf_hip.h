@@ -205,7 +205,9 @@ bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_
                         void* S0_out);
 
 /* Evaluates the device digamma forms on the GPU (kind 0: FP32 bulk psi,
- * 1: exact FP64, 2: util::psi_appr in FP64, src/util/util.hpp:440-514).
+ * 1: exact FP64, 2: util::psi_appr in FP64, src/util/util.hpp:440-514,
+ * 3: FP32 psi(s+1) of the generic update pass at x = s+1, 4: FP32 psi of
+ * the exact-z update pass (log2 units, times ln 2), x >= 1).
  * x and out are host arrays of n doubles. */
 bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n);
 

@@ -61,8 +61,10 @@ def test_device_psi_fp32(hip, oracle, golden):
     got = hip.psi_eval(0, xs)
     assert np.all(np.abs(got - ref) <= 6e-7 * np.maximum(1, np.abs(ref)))
     hot = xr[xr >= 1]
-    got_hot = hip.psi_eval(3, hot)
-    assert np.all(np.abs(got_hot - oracle.psi_vec(hot)) <= 6e-7 * np.maximum(1, np.abs(oracle.psi_vec(hot))))
+    ref_hot = oracle.psi_vec(hot)
+    for kind in (3, 4):
+        got_hot = hip.psi_eval(kind, hot)
+        assert np.all(np.abs(got_hot - ref_hot) <= 6e-7 * np.maximum(1, np.abs(ref_hot))), kind
 
 
 # ------------------------------------------------------------------ FP64
