@@ -71,7 +71,11 @@ struct F32Layout {
 // prefetch has a fixed instruction count.
 template <int Z>
 struct F32zLds {
-    static constexpr uint32_t CPL = (64 * Z + 8 + 255) / 256;  // 16-byte chunks per lane
+    // A row segment starts d floats past a 16-byte boundary: d = 0 when
+    // z % 4 == 0, d in {0, 2} when z is even, else d <= 3. It spans
+    // ceil((d + 64 z) / 4) 16-byte chunks.
+    static constexpr uint32_t DMAX = (Z % 4 == 0) ? 0u : ((Z % 2 == 0) ? 2u : 3u);
+    static constexpr uint32_t CPL = (16u * Z + (DMAX ? 1u : 0u) + 63u) / 64u;  // chunks per lane
     static constexpr size_t psiA_off = round16((size_t)64 * Z * 4);
     static constexpr size_t gm_off = psiA_off + round16((size_t)Z * 4);
     static constexpr size_t gm_stride = round16((size_t)Z * 4);
@@ -90,7 +94,7 @@ constexpr int f32z_waves() {
 
 // Waves per SIMD the register allocation must leave room for.
 #ifndef BNMF_F32Z_WPE
-#define BNMF_F32Z_WPE(Z) ((Z) <= 12 ? 4 : (Z) <= 20 ? 3 : (Z) <= 37 ? 2 : 1)
+#define BNMF_F32Z_WPE(Z) ((Z) <= 12 ? 4 : (Z) <= 20 ? 3 : (Z) <= 36 ? 2 : 1)
 #endif
 
 // Memory pipelining: a wave issues the next row's loads (S chunks, X, psia)
