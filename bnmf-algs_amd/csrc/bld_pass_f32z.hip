@@ -76,7 +76,14 @@ struct F32zLds {
     // ceil((d + 64 z) / 4) 16-byte chunks.
     static constexpr uint32_t DMAX = (Z % 4 == 0) ? 0u : ((Z % 2 == 0) ? 2u : 3u);
     static constexpr uint32_t CPL = (16u * Z + (DMAX ? 1u : 0u) + 63u) / 64u;  // chunks per lane
-    static constexpr size_t psiA_off = round16((size_t)64 * Z * 4);
+    // psi(beta_eph) rows of the tile: stride Z when z % 4 == 0 (read with
+    // the fiber's own chunk order); otherwise padded to a multiple of 4
+    // floats with an odd number of 16-byte groups, so each lane reads it
+    // 16 bytes at a time with the lanes of an access cycle on distinct
+    // banks, whatever z is
+    static constexpr uint32_t PBS = (Z % 4 == 0) ? (uint32_t)Z
+                                    : (((Z + 3) / 4) % 2 == 1 ? (Z + 3) / 4 * 4 : (Z + 3) / 4 * 4 + 4);
+    static constexpr size_t psiA_off = round16((size_t)64 * PBS * 4);
     static constexpr size_t gm_off = psiA_off + round16((size_t)Z * 4);
     static constexpr size_t gm_stride = round16((size_t)Z * 4);
     static constexpr size_t raw_stride = (size_t)CPL * 64 * 16;
@@ -94,7 +101,7 @@ constexpr int f32z_waves() {
 
 // Waves per SIMD the register allocation must leave room for.
 #ifndef BNMF_F32Z_WPE
-#define BNMF_F32Z_WPE(Z) ((Z) <= 12 ? 4 : (Z) <= 20 ? 3 : (Z) <= 36 ? 2 : 1)
+#define BNMF_F32Z_WPE(Z) ((Z) <= 12 ? 4 : (Z) <= 20 ? 3 : ((Z) <= 32 || (Z) == 36) ? 2 : 1)
 #endif
 
 // Memory pipelining: a wave issues the next row's loads (S chunks, X, psia)
@@ -121,6 +128,43 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     typedef float V16 __attribute__((ext_vector_type(4)));
     typedef uint32_t U16 __attribute__((ext_vector_type(4)));
     typedef typename VecR<float, RW>::type VR;
+    typedef float V8 __attribute__((ext_vector_type(2)));
+    constexpr bool PAD = (Z % 4) != 0;
+    constexpr int NG = (Z + 3) / 4;  // groups of 4 elements of a fiber
+    // elements 4*g4 .. 4*g4+3 (those below Z) of a fiber in RW-wide LDS
+    // accesses; fb is RW-aligned (z even: RW = 2, d even)
+    auto load4 = [](const float* fb, int g4, float* s4) {
+#pragma unroll
+        for (int v = 0; v < 4; v += RW) {
+            if (4 * g4 + v < Z) {
+                if constexpr (RW == 2) {
+                    const V8 t = *reinterpret_cast<const V8*>(fb + 4 * g4 + v);
+                    s4[v] = t[0];
+                    s4[v + 1] = t[1];
+                } else {
+                    s4[v] = fb[4 * g4 + v];
+                }
+            } else {
+                s4[v] = 0.0f;
+                if constexpr (RW == 2) s4[v + 1] = 0.0f;
+            }
+        }
+    };
+    auto store4 = [](float* fb, int g4, const float* s4) {
+#pragma unroll
+        for (int v = 0; v < 4; v += RW) {
+            if (4 * g4 + v < Z) {
+                if constexpr (RW == 2) {
+                    V8 t;
+                    t[0] = s4[v];
+                    t[1] = s4[v + 1];
+                    *reinterpret_cast<V8*>(fb + 4 * g4 + v) = t;
+                } else {
+                    fb[4 * g4 + v] = s4[v];
+                }
+            }
+        }
+    };
 
     const uint32_t n_items = *n_items_p;
     const uint32_t tid = threadIdx.x;
@@ -189,8 +233,13 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
             for (uint32_t u = 0; u < PB; ++u) pbv[u] = psib_g[min(tid + u * NT, n_el - 1u)];
 #pragma unroll
-            for (uint32_t u = 0; u < PB; ++u)
-                if (tid + u * NT < n_el) psib_l[tid + u * NT] = pbv[u] * kInvLn2;
+            for (uint32_t u = 0; u < PB; ++u) {
+                const uint32_t e = tid + u * NT;
+                if (e < n_el) {
+                    const uint32_t f = e / (uint32_t)Z, k = e - f * (uint32_t)Z;
+                    psib_l[f * Lds::PBS + k] = pbv[u] * kInvLn2;
+                }
+            }
             // psi(sum_i alpha_eph(i,k)) (update_grad_minus,
             // bld_mult_funcs.hpp:213-220), from the prep launch
             if (tid < Z) psiA_l[tid] = gconst<float>(I->psiA)[tid];
@@ -224,7 +273,7 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
             const float xv = lane < nj ? xs_l[lane] : 0.0f;
             if (lane < nj) {
                 float* const fb = raw + cur_dl + lane * Z;
-                const float* const pbb = psib_l + lane * Z;
+                const float* const pbb = psib_l + lane * Lds::PBS;
                 float gp[Z];
                 float sv_keep[KEEP ? Z : 1];
                 float sumS = 0.0f, nom = 0.0f, den = 0.0f;
@@ -232,21 +281,38 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                 // units) and the fiber sums S_ij+, sum_k S*gm and sum_k S*g.
                 // S >= 0 throughout (S0 >= 0 and beta >= 1), so S + 1 >= 1
                 // and psi takes its x >= 1 form directly.
+                auto elem_a = [&](int k, float sk, float pbk, float gmk) {
+                    if constexpr (KEEP) sv_keep[k] = sk;
+                    const float g = pbk - psi_ge1_log2_f32(sk + 1.0f);
+                    gp[k] = g;
+                    sumS += sk;
+                    nom = fmaf(sk, gmk, nom);
+                    den = fmaf(sk, g, den);
+                };
+                if constexpr (!PAD) {
 #pragma unroll
-                for (int m = 0; m < NCH; ++m) {
-                    const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
-                    const VR sv = *reinterpret_cast<const VR*>(fb + mm);
-                    const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
-                    const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+                    for (int m = 0; m < NCH; ++m) {
+                        const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                        const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                        const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                        const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
 #pragma unroll
-                    for (int v = 0; v < RW; ++v) {
-                        const float sk = vget<float, RW>(sv, v);
-                        if constexpr (KEEP) sv_keep[m * RW + v] = sk;
-                        const float g = vget<float, RW>(pb, v) - psi_ge1_log2_f32(sk + 1.0f);
-                        gp[m * RW + v] = g;
-                        sumS += sk;
-                        nom = fmaf(sk, vget<float, RW>(gv, v), nom);
-                        den = fmaf(sk, g, den);
+                        for (int v = 0; v < RW; ++v)
+                            elem_a(m * RW + v, vget<float, RW>(sv, v), vget<float, RW>(pb, v),
+                                   vget<float, RW>(gv, v));
+                    }
+                } else {
+                    // z % 4 != 0: the fiber in RW-wide reads, psi(beta_eph)
+                    // and gm 16 bytes at a time
+#pragma unroll
+                    for (int g4 = 0; g4 < NG; ++g4) {
+                        const V16 pb = *reinterpret_cast<const V16*>(pbb + 4 * g4);
+                        const V16 gv = *reinterpret_cast<const V16*>(gm_l + 4 * g4);
+                        float s4[4];
+                        load4(fb, g4, s4);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (4 * g4 + v < Z) elem_a(4 * g4 + v, s4[v], pb[v], gv[v]);
                     }
                 }
                 // Per-fiber constants. With Xh = 1/(X + eps) (X_reciprocal)
@@ -264,26 +330,46 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                 const float q = upd ? (sumS + epsf) / xv : 0.0f;
                 const float dq = upd ? (den + epsf) * q : __builtin_inff();
                 // Phase B: the multiplicative update and the S_+jk partials
+                auto elem_b = [&](int k, float sk, float gmk) {
+                    const float r = recip_fast<float>(fmaf(gmk, q, dq));
+                    const float sn = sk * ((gp[k] + nom) * r);
+                    acc[k] += sn;
+                    return sn;
+                };
+                if constexpr (!PAD) {
 #pragma unroll
-                for (int m = 0; m < NCH; ++m) {
-                    const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
-                    VR sv;
-                    if constexpr (KEEP) {
+                    for (int m = 0; m < NCH; ++m) {
+                        const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                        VR sv;
+                        if constexpr (KEEP) {
 #pragma unroll
-                        for (int v = 0; v < RW; ++v) vset<float, RW>(sv, v, sv_keep[m * RW + v]);
-                    } else {
-                        sv = *reinterpret_cast<const VR*>(fb + mm);
+                            for (int v = 0; v < RW; ++v) vset<float, RW>(sv, v, sv_keep[m * RW + v]);
+                        } else {
+                            sv = *reinterpret_cast<const VR*>(fb + mm);
+                        }
+                        const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+#pragma unroll
+                        for (int v = 0; v < RW; ++v)
+                            vset<float, RW>(sv, v, elem_b(m * RW + v, vget<float, RW>(sv, v),
+                                                          vget<float, RW>(gv, v)));
+                        *reinterpret_cast<VR*>(fb + mm) = sv;
                     }
-                    const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+                } else {
 #pragma unroll
-                    for (int v = 0; v < RW; ++v) {
-                        const float sk = vget<float, RW>(sv, v);
-                        const float r = recip_fast<float>(fmaf(vget<float, RW>(gv, v), q, dq));
-                        const float sn = sk * ((gp[m * RW + v] + nom) * r);
-                        vset<float, RW>(sv, v, sn);
-                        acc[m * RW + v] += sn;
+                    for (int g4 = 0; g4 < NG; ++g4) {
+                        const V16 gv = *reinterpret_cast<const V16*>(gm_l + 4 * g4);
+                        float s4[4];
+                        if constexpr (KEEP) {
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) s4[v] = 4 * g4 + v < Z ? sv_keep[4 * g4 + v] : 0.0f;
+                        } else {
+                            load4(fb, g4, s4);
+                        }
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (4 * g4 + v < Z) s4[v] = elem_b(4 * g4 + v, s4[v], gv[v]);
+                        store4(fb, g4, s4);
                     }
-                    *reinterpret_cast<VR*>(fb + mm) = sv;
                 }
             }
             wave_sync();
