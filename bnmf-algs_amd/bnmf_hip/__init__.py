@@ -118,6 +118,11 @@ def lib():
         L.bnmf_init_S.restype = st
         L.bnmf_psi_eval.argtypes = [ctypes.c_int, _D, _D, _SZ]
         L.bnmf_psi_eval.restype = st
+        L.bnmf_batch_fact.argtypes = [_VP, _SZ, ctypes.c_double, _D, _D, _D]
+        L.bnmf_batch_fact.restype = st
+        L.bnmf_bld_fact.argtypes = [ctypes.c_int, _VP, _SZ, _SZ, _SZ, _D, _D, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, _D, _D, _D]
+        L.bnmf_bld_fact.restype = st
         L.bnmf_tune_set.argtypes = [ctypes.c_int, _SZ, _SZ, ctypes.c_uint]
         L.bnmf_tune_set.restype = st
         L.bnmf_tune_get.argtypes = [ctypes.c_int, _SZ, _SZ, ctypes.POINTER(ctypes.c_uint)]
@@ -308,10 +313,27 @@ def init_S(seed: int, X, z: int, dtype=np.float64) -> np.ndarray:
     return S
 
 
+def bld_fact(S, model_params: Params, eps: float = 1e-50):
+    """bld::bld_fact (src/bld/bld_fact.hpp:36-80): W (x, z), H (z, y), L (y,)
+    from S, with the axis sums and normalisation on the GPU."""
+    S = np.asarray(S)
+    dt = np.dtype(np.float32) if S.dtype == np.float32 else np.dtype(np.float64)
+    S = np.ascontiguousarray(S, dtype=dt)
+    x, y, z = S.shape
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    if alpha.shape != (x,) or beta.shape != (z,):
+        raise ValueError("alpha must have x entries and beta z entries")
+    W, H, L = np.empty((x, z)), np.empty((z, y)), np.empty(y)
+    _check(lib().bnmf_bld_fact(_code(dt), _ptr(S), x, y, z, _dptr(alpha), _dptr(beta),
+                               float(model_params.a), float(model_params.b), float(eps),
+                               _dptr(W), _dptr(H), _dptr(L)), "bld_fact")
+    return W, H, L
+
+
 def psi_eval(kind: int, x) -> np.ndarray:
     """Device digamma forms (0: FP32 bulk, 1: exact FP64, 2: psi_appr FP64,
-    3: FP32 psi(s+1) of the generic update pass at x = s+1, 4: FP32 psi of the
-    exact-z update pass, x >= 1)."""
+    3: FP32 psi(s+1) of the update passes evaluated at x = s+1)."""
     x = np.ascontiguousarray(x, dtype=np.float64).ravel()
     out = np.empty_like(x)
     _check(lib().bnmf_psi_eval(int(kind), _dptr(x), _dptr(out), x.size), "psi_eval")
@@ -387,6 +409,14 @@ class Batch:
         out = np.zeros(max(1, len(self.shapes)))
         _check(lib().bnmf_batch_log_marginal(self._h, _dptr(out)), "batch_log_marginal")
         return out[:len(self.shapes)]
+
+    def fact(self, idx: int, eps: float = 1e-50):
+        """bld::bld_fact of instance idx's current S: (W (x, z), H (z, y), L (y,))."""
+        x, y, z = self.shapes[idx]
+        W, H, L = np.empty((x, z)), np.empty((z, y)), np.empty(y)
+        _check(lib().bnmf_batch_fact(self._h, int(idx), float(eps), _dptr(W), _dptr(H), _dptr(L)),
+               "batch_fact")
+        return W, H, L
 
     def time_pass(self, reps: int = 20):
         p, it = np.zeros(1), np.zeros(1)

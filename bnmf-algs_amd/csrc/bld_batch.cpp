@@ -436,6 +436,7 @@ void Batch::capture_graphs() {
 }
 
 void Batch::solve_async(size_t iters) {
+    partials_valid_ = false;
     if (n_prep_items_ == 0) return;
     if (iters == 0) {
         // S = S0 for every instance
@@ -448,6 +449,7 @@ void Batch::solve_async(size_t iters) {
         }
         return;
     }
+    partials_valid_ = false;
     // partial axis sums of S0, then the first iteration reading S0
     enqueue_pass(MODE_SUMS, true);
     enqueue_iteration(true);
@@ -461,6 +463,7 @@ void Batch::solve_async(size_t iters) {
         BNMF_HIP(hipGraphLaunch(graph1_, stream_));
         --rem;
     }
+    partials_valid_ = true;
 }
 
 void Batch::synchronize() { BNMF_HIP(hipStreamSynchronize(stream_)); }
@@ -476,6 +479,7 @@ void Batch::get_S(size_t idx, void* host) const {
 }
 
 void Batch::log_marginal(double* out) {
+    partials_valid_ = false;
     if (n_prep_items_ > 0) {
         enqueue_pass(MODE_SCORE, false);
         char* m = meta_.as<char>();
@@ -503,6 +507,7 @@ void Batch::time_pass(int reps, double* pass_ms, double* iter_ms) {
         *pass_ms = *iter_ms = 0.0;
         return;
     }
+    partials_valid_ = false;
     // Two graphs, launched the way solves launch them: reps update passes
     // (every pass launch of an iteration, forked over the side streams),
     // and reps whole iterations (prep + pass). Each is replayed once to
@@ -564,6 +569,7 @@ void Batch::time_pass(int reps, double* pass_ms, double* iter_ms) {
 
 void Batch::tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp) {
     BNMF_REQUIRE(layouts_.size() == 1, "tensor_sums works on a single instance");
+    partials_valid_ = false;
     const InstLayout& L = layouts_[0];
     if (L.elements() == 0) return;
     // sums of S (the current tensor), fiber sums into a scratch buffer
@@ -584,6 +590,33 @@ void Batch::tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp) {
     if (S_pjk) BNMF_HIP(hipMemcpy(S_pjk, arena_.as<char>(L.off_psib), (size_t)L.y * L.z * es_, hipMemcpyDeviceToHost));
     if (S_ipk) BNMF_HIP(hipMemcpy(S_ipk, arena_.as<char>(L.off_psia), (size_t)L.x * L.z * es_, hipMemcpyDeviceToHost));
     if (S_ijp) BNMF_HIP(hipMemcpy(S_ijp, sijp.ptr, (size_t)L.x * L.y * es_, hipMemcpyDeviceToHost));
+}
+
+void Batch::fact(size_t idx, double eps, double* W, double* H, double* L) {
+    BNMF_REQUIRE(idx < layouts_.size(), "instance index out of range");
+    const InstLayout& Li = layouts_[idx];
+    if (Li.elements() == 0) return;
+    char* m = meta_.as<char>();
+    if (!partials_valid_) {
+        // the partials of the current S (one read of S)
+        enqueue_pass(MODE_SUMS, false);
+        partials_valid_ = true;
+    }
+    // raw prep: S_+jk and S_i+k into psib / psia (the next solve starts
+    // from S0 and recomputes them); the partials stay as they were
+    BNMF_HIP(launch_prep(dtype_, psi_, true, dim3(n_prep_items_), stream_,
+                         reinterpret_cast<const BldInst*>(m + off_desc_),
+                         reinterpret_cast<const PrepItem*>(m + off_prep_items_),
+                         reinterpret_cast<const uint32_t*>(m + off_prep_count_), (uint32_t)layouts_.size()));
+    const size_t nW = (size_t)Li.x * Li.z, nH = (size_t)Li.z * Li.y, nL = Li.y;
+    DeviceBuffer out((nW + nH + nL) * 8);
+    const BldInst* d = reinterpret_cast<const BldInst*>(m + off_desc_) + idx;
+    BNMF_HIP(launch_fact(dtype_, d, Li.y, Li.z, eps, out.as<double>(), out.as<double>(nW * 8),
+                         out.as<double>((nW + nH) * 8), stream_));
+    BNMF_HIP(hipStreamSynchronize(stream_));
+    if (W) BNMF_HIP(hipMemcpy(W, out.as<double>(), nW * 8, hipMemcpyDeviceToHost));
+    if (H) BNMF_HIP(hipMemcpy(H, out.as<double>(nW * 8), nH * 8, hipMemcpyDeviceToHost));
+    if (L) BNMF_HIP(hipMemcpy(L, out.as<double>((nW + nH) * 8), nL * 8, hipMemcpyDeviceToHost));
 }
 
 bnmf_batch_info Batch::info() const {

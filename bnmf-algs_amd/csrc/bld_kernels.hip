@@ -738,6 +738,55 @@ hipError_t launch_prep(int dtype, int psi, bool raw, dim3 grid, hipStream_t st,
     return hipGetLastError();
 }
 
+// bld::bld_fact (src/bld/bld_fact.hpp:36-80) from the axis sums of S, which
+// a raw prep has left in the descriptor's psib (S_+jk, y x z) and psia
+// (S_i+k, x x z) buffers. Blocks 0..z-1: column k of W, normalised by its
+// sum over i; blocks z..: 256 columns j of H (normalised over k) and L_j.
+// Sums in double, per-thread strided partials added in a fixed tree.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    bld_fact_kernel(const BldInst* __restrict__ I, double eps, double* __restrict__ W,
+                    double* __restrict__ H, double* __restrict__ L) {
+    __shared__ double red[4];
+    const uint32_t x = I->x, y = I->y, z = I->z, tid = threadIdx.x;
+    const T* __restrict__ S_pjk = static_cast<const T*>(I->psib);
+    const T* __restrict__ S_ipk = static_cast<const T*>(I->psia);
+    if (blockIdx.x < z) {
+        const uint32_t k = blockIdx.x;
+        double part = 0.0;
+        for (uint32_t i = tid; i < x; i += 256) part += I->alpha[i] + (double)S_ipk[(size_t)i * z + k] - 1.0;
+        const double cs = block_sum_256(part, red);
+        __shared__ double colsum;
+        if (tid == 0) colsum = cs + eps;
+        __syncthreads();
+        for (uint32_t i = tid; i < x; i += 256)
+            W[(size_t)i * z + k] = (I->alpha[i] + (double)S_ipk[(size_t)i * z + k] - 1.0) / colsum;
+    } else {
+        const uint32_t j = (blockIdx.x - z) * 256 + tid;
+        if (j >= y) return;
+        double hs = 0.0, sp = 0.0;
+        for (uint32_t k = 0; k < z; ++k) {
+            const double s = (double)S_pjk[(size_t)j * z + k];
+            hs += I->beta[k] + s - 1.0;
+            sp += s;
+        }
+        L[j] = (I->a + sp - 1.0) / (I->b + 1.0 + eps);
+        hs += eps;
+        for (uint32_t k = 0; k < z; ++k)
+            H[(size_t)k * y + j] = (I->beta[k] + (double)S_pjk[(size_t)j * z + k] - 1.0) / hs;
+    }
+}
+
+hipError_t launch_fact(int dtype, const BldInst* inst, uint32_t y, uint32_t z, double eps, double* W,
+                       double* H, double* L, hipStream_t st) {
+    const dim3 grid(z + (y + 255) / 256);
+    if (dtype == 0)
+        hipLaunchKernelGGL(bld_fact_kernel<float>, grid, dim3(256), 0, st, inst, eps, W, H, L);
+    else
+        hipLaunchKernelGGL(bld_fact_kernel<double>, grid, dim3(256), 0, st, inst, eps, W, H, L);
+    return hipGetLastError();
+}
+
 // Element-wise evaluation of the device digamma forms (test hook).
 __global__ void __launch_bounds__(256)
     bld_psi_eval_kernel(int kind, const double* __restrict__ xin, double* __restrict__ out, size_t n) {
@@ -750,8 +799,6 @@ __global__ void __launch_bounds__(256)
         r = (double)psi_f32(f);
     } else if (kind == 3) {
         r = (double)psi_sp1<PSI_F32, float>((float)v - 1.0f);
-    } else if (kind == 4) {
-        r = (double)psi_ge1_log2_f32((float)v) * 0.6931471805599453;
     } else if (kind == 1) {
         r = psi_f64(v);
     } else {

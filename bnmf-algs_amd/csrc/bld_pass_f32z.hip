@@ -32,7 +32,6 @@ namespace bnmf {
 // GPUs, and an offset past any resource of this kernel: a store given it is
 // dropped by the range check.
 constexpr int kBufWord3 = 0x00020000;
-constexpr float kInvLn2 = 1.4426950408889634f;
 constexpr uint32_t kBufDrop = 0x80000000u;
 
 // Compile-time form of read_layout() for float: LDS read width RW and the XOR
@@ -225,9 +224,6 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 
         {
             const BNMF_GLOBAL float* __restrict__ psib_g = gconst<float>(I->psib) + (size_t)j0 * Z;
-            // psi values enter the pass in log2 units (psi / ln 2): the
-            // update is a ratio of psi differences, so the scale cancels,
-            // and psi(S+1) then needs no final multiply by ln 2
             constexpr uint32_t PB = (64u * Z + NT - 1) / NT;  // elements per thread
             float pbv[PB];
 #pragma unroll
@@ -237,7 +233,7 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                 const uint32_t e = tid + u * NT;
                 if (e < n_el) {
                     const uint32_t f = e / (uint32_t)Z, k = e - f * (uint32_t)Z;
-                    psib_l[f * Lds::PBS + k] = pbv[u] * kInvLn2;
+                    psib_l[f * Lds::PBS + k] = pbv[u];
                 }
             }
             // psi(sum_i alpha_eph(i,k)) (update_grad_minus,
@@ -259,7 +255,7 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
         auto row_step = [&](uint32_t i) {
 #pragma unroll
             for (int c = 0; c < CPL; ++c) reinterpret_cast<V16*>(raw)[(uint32_t)c * 64u + lane] = pf[c];
-            if (lane < (uint32_t)Z) gm_l[lane] = (psiA_l[lane] - pf_pa) * kInvLn2;
+            if (lane < (uint32_t)Z) gm_l[lane] = psiA_l[lane] - pf_pa;
             // X_ij goes through LDS too: the prefetch register is then free
             // before the next row's load, and the loop carries no copy that
             // would have to wait for that load
@@ -277,13 +273,17 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                 float gp[Z];
                 float sv_keep[KEEP ? Z : 1];
                 float sumS = 0.0f, nom = 0.0f, den = 0.0f;
-                // Phase A: per element g = psi(beta_eph) - psi(S+1) (in log2
-                // units) and the fiber sums S_ij+, sum_k S*gm and sum_k S*g.
+                // Phase A: per element g = psi(beta_eph) - psi(S+1) and the
+                // fiber sums S_ij+, sum_k S*gm and sum_k S*g.
                 // S >= 0 throughout (S0 >= 0 and beta >= 1), so S + 1 >= 1
                 // and psi takes its x >= 1 form directly.
                 auto elem_a = [&](int k, float sk, float pbk, float gmk) {
                     if constexpr (KEEP) sv_keep[k] = sk;
-                    const float g = pbk - psi_ge1_log2_f32(sk + 1.0f);
+                    // psi(S+1) rounded at its own magnitude first: forming
+                    // pbk - ln2*log2(S+1) before adding the small polynomial
+                    // term would round at the (larger) magnitude of
+                    // psi(beta_eph), which ill-conditioned instances amplify
+                    const float g = pbk - psi_ge1_f32(sk + 1.0f);
                     gp[k] = g;
                     sumS += sk;
                     nom = fmaf(sk, gmk, nom);

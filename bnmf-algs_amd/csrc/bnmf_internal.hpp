@@ -55,6 +55,8 @@ hipError_t launch_psi_eval(int kind, const double* xin, double* out, size_t n, h
 hipError_t launch_init_s(int dtype, void* S, const void* X, size_t n_fibers, uint32_t z,
                          uint64_t seed, hipStream_t st);
 hipError_t init_kernel_attributes();
+hipError_t launch_fact(int dtype, const BldInst* inst, uint32_t y, uint32_t z, double eps, double* W,
+                       double* H, double* L, hipStream_t st);
 // FP32 update pass specialised on the exact fiber depth z (bld_pass_f32z.hip)
 hipError_t launch_pass_f32z(int z, dim3 grid, hipStream_t st, const BldInst* insts,
                             const PassItem* items, const uint32_t* n_items);
@@ -158,6 +160,8 @@ class Batch {
     void log_marginal(double* out);
     void time_pass(int reps, double* pass_ms, double* iter_ms);
     void tensor_sums(void* S_pjk, void* S_ipk, void* S_ijp);
+    // bld_fact of instance idx's current S: W (x*z), H (z*y), L (y), host
+    void fact(size_t idx, double eps, double* W, double* H, double* L);
     bnmf_batch_info info() const;
     size_t size() const { return layouts_.size(); }
 
@@ -197,6 +201,9 @@ class Batch {
     std::vector<hipEvent_t> fork_ev_, join_ev_;
     hipGraphExec_t graph1_ = nullptr, graphU_ = nullptr;
     int unroll_ = 8;
+    // the partial axis sums in the workspace belong to the current S (true
+    // after a solve: its last pass accumulates them)
+    bool partials_valid_ = false;
     int cus_ = 256;
     size_t device_bytes_ = 0;
 };

@@ -246,6 +246,40 @@ bnmf_status bnmf_batch_log_marginal(bnmf_batch* b, double* out) {
     });
 }
 
+bnmf_status bnmf_batch_fact(bnmf_batch* b, size_t idx, double eps, double* W, double* H, double* L) {
+    return guarded([&] {
+        BNMF_REQUIRE(b != nullptr, "null argument");
+        b->impl->fact(idx, eps, W, H, L);
+    });
+}
+
+bnmf_status bnmf_bld_fact(int dtype, const void* S, size_t x, size_t y, size_t z, const double* alpha,
+                          const double* beta, double a, double b, double eps, double* W, double* H,
+                          double* L) {
+    return guarded([&] {
+        check_dtype(dtype);
+        BNMF_REQUIRE(x * y * z == 0 || S != nullptr, "S is null");
+        BNMF_REQUIRE(x == 0 || alpha != nullptr, "alpha is null");
+        BNMF_REQUIRE(beta != nullptr && z >= 1 && z <= bnmf::kMaxZ, "beta is null or z out of range");
+        BNMF_REQUIRE(x < (1u << 31) && y < (1u << 31), "x and y must be below 2^31");
+        if (x * y * z == 0) return;
+        // X is not read by the sums pass; a zero matrix satisfies the layout
+        std::vector<char> Xz(x * y * esize(dtype), 0);
+        bnmf::InstSpec s;
+        s.x = (uint32_t)x;
+        s.y = (uint32_t)y;
+        s.z = (uint32_t)z;
+        s.X = Xz.data();
+        s.alpha = alpha;
+        s.beta = beta;
+        s.a = a;
+        s.b = b;
+        s.S0 = S;
+        bnmf::Batch bt(dtype, dtype == BNMF_F32 ? bnmf::PSIM_F32 : bnmf::PSIM_F64_EXACT, 1e-50, {s});
+        bt.fact(0, eps, W, H, L);
+    });
+}
+
 bnmf_status bnmf_batch_time_pass(bnmf_batch* b, int reps, double* pass_ms, double* iter_ms) {
     return guarded([&] {
         BNMF_REQUIRE(b != nullptr && pass_ms != nullptr && iter_ms != nullptr, "null argument");
@@ -294,7 +328,7 @@ bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_
 
 bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n) {
     return guarded([&] {
-        BNMF_REQUIRE(kind >= 0 && kind <= 4, "kind must be 0..4");
+        BNMF_REQUIRE(kind >= 0 && kind <= 3, "kind must be 0..3");
         BNMF_REQUIRE(n == 0 || (x && out), "null argument");
         if (n == 0) return;
         bnmf::ensure_runtime();

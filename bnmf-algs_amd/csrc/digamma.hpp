@@ -148,33 +148,6 @@ __device__ inline float psi_ge1_f32(float x) {
 __host__ __device__ inline float psi_ge1_f32(float x) { return logf(x) + psi32_q(1.0f / x); }
 #endif
 
-// psi(x) / ln 2 for x >= 1: log2(x) + u Q(u)/ln 2 with the coefficients
-// pre-divided by ln 2, one FMA shorter than psi_ge1_f32 (the exact-z update
-// pass works in these units).
-#define BNMF_INV_LN2 1.4426950408889634
-__host__ __device__ inline float psi32_q_log2(float u) {
-    float p = (float)(BNMF_PSI32_Q8 * BNMF_INV_LN2);
-    p = fmaf(p, u, (float)(BNMF_PSI32_Q7 * BNMF_INV_LN2));
-    p = fmaf(p, u, (float)(BNMF_PSI32_Q6 * BNMF_INV_LN2));
-    p = fmaf(p, u, (float)(BNMF_PSI32_Q5 * BNMF_INV_LN2));
-    p = fmaf(p, u, (float)(BNMF_PSI32_Q4 * BNMF_INV_LN2));
-    p = fmaf(p, u, (float)(BNMF_PSI32_Q3 * BNMF_INV_LN2));
-    p = fmaf(p, u, (float)(BNMF_PSI32_Q2 * BNMF_INV_LN2));
-    p = fmaf(p, u, (float)(BNMF_PSI32_Q1 * BNMF_INV_LN2));
-    return p;
-}
-#if defined(__HIP_DEVICE_COMPILE__)
-__device__ inline float psi_ge1_log2_f32(float x) {
-    const float u = __builtin_amdgcn_rcpf(x);
-    return fmaf(u, psi32_q_log2(u), __builtin_amdgcn_logf(x));
-}
-#else
-__host__ __device__ inline float psi_ge1_log2_f32(float x) {
-    const float u = 1.0f / x;
-    return fmaf(u, psi32_q_log2(u), log2f(x));
-}
-#endif
-
 // any x > 0 (prep arguments S_i+k + alpha_i, S_+jk + beta_k, their column sums)
 __host__ __device__ inline float psi_f32(float x) {
     float shift = 0.0f;

@@ -1,11 +1,13 @@
-// bld_fact.hpp — W, H, L from S (role of src/bld/bld_fact.hpp:36-80): the
-// axis sums come from the GPU, the O((x+y)z) normalisation runs here.
+// bld_fact.hpp — W, H, L from S (src/bld/bld_fact.hpp:36-80): the axis sums
+// and the normalisation run on the GPU (bnmf_bld_fact); results in T.
 #pragma once
 
 #include <tuple>
+#include <vector>
 
 #include "../alloc_model/alloc_model_params.hpp"
-#include "../util/util.hpp"
+#include "../defs.hpp"
+#include "../detail.hpp"
 
 namespace bnmf_algs {
 namespace bld {
@@ -18,32 +20,19 @@ std::tuple<matrix_t<T>, matrix_t<T>, vector_t<T>> bld_fact(const tensor_t<T, 3>&
     BNMF_ASSERT(p.alpha.size() == static_cast<size_t>(x),
                 "Number of alpha parameters must be equal to S.dimension(0)");
     BNMF_ASSERT(p.beta.size() == static_cast<size_t>(z), "Number of beta parameters must be equal to z");
-    const auto sums = util::tensor_sums(S);
-    const tensor_t<T, 2>& S_pjk = sums[0];
-    const tensor_t<T, 2>& S_ipk = sums[1];
+    const std::vector<double> alpha = bnmf_algs::details::to_double(p.alpha);
+    const std::vector<double> beta = bnmf_algs::details::to_double(p.beta);
+    std::vector<double> w((size_t)(x * z)), h((size_t)(z * y)), l((size_t)y);
+    bnmf_algs::details::check_status(
+        bnmf_bld_fact(bnmf_algs::details::dtype_of<T>(), S.data(), (size_t)x, (size_t)y, (size_t)z,
+                      alpha.data(), beta.data(), (double)p.a, (double)p.b, eps, w.data(), h.data(),
+                      l.data()),
+        "bld_fact");
     matrix_t<T> W(x, z), H(z, y);
     vector_t<T> L(1, y);
-    for (long i = 0; i < x; ++i)
-        for (long k = 0; k < z; ++k) W(i, k) = p.alpha[i] + S_ipk(i, k) - 1;
-    for (long k = 0; k < z; ++k)
-        for (long j = 0; j < y; ++j) H(k, j) = p.beta[k] + S_pjk(j, k) - 1;
-    for (long j = 0; j < y; ++j) {
-        T s = 0;
-        for (long k = 0; k < z; ++k) s += S_pjk(j, k);
-        L(0, j) = (p.a + s - 1) / (p.b + 1 + eps);
-    }
-    for (long k = 0; k < z; ++k) {
-        T cs = 0;
-        for (long i = 0; i < x; ++i) cs += W(i, k);
-        cs += eps;
-        for (long i = 0; i < x; ++i) W(i, k) /= cs;
-    }
-    for (long j = 0; j < y; ++j) {
-        T cs = 0;
-        for (long k = 0; k < z; ++k) cs += H(k, j);
-        cs += eps;
-        for (long k = 0; k < z; ++k) H(k, j) /= cs;
-    }
+    for (long t = 0; t < x * z; ++t) W.data()[t] = static_cast<T>(w[(size_t)t]);
+    for (long t = 0; t < z * y; ++t) H.data()[t] = static_cast<T>(h[(size_t)t]);
+    for (long t = 0; t < y; ++t) L.data()[t] = static_cast<T>(l[(size_t)t]);
     return std::make_tuple(W, H, L);
 }
 

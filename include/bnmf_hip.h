@@ -69,6 +69,13 @@ bnmf_status bnmf_log_marginal_S(int dtype, const void* S, size_t x, size_t y, si
                                 const double* alpha, const double* beta, double a,
                                 double b, double* out);
 
+/* bld::bld_fact (src/bld/bld_fact.hpp:36-80) of a host tensor S (x*y*z of
+ * the given dtype): W (x*z), H (z*y), L (y) in double; axis sums and
+ * normalisation on the GPU. */
+bnmf_status bnmf_bld_fact(int dtype, const void* S, size_t x, size_t y, size_t z, const double* alpha,
+                          const double* beta, double a, double b, double eps, double* W, double* H,
+                          double* L);
+
 /* cuda::tensor_sums (src/cuda/tensor_ops.hpp:40-42, cuda_src/tensor_ops.cu:10-55):
  * S_pjk (y*z) = sum over axis 0, S_ipk (x*z) = axis 1, S_ijp (x*y) = axis 2.
  * Any output may be NULL. */
@@ -111,6 +118,12 @@ bnmf_status bnmf_batch_log_marginal(bnmf_batch* b, double* out);
  * with HIP events on the batch stream. Advances S by 4*reps passes; solve
  * again (from S0) to restore a result. */
 bnmf_status bnmf_batch_time_pass(bnmf_batch* b, int reps, double* pass_ms, double* iter_ms);
+
+/* bld::bld_fact (src/bld/bld_fact.hpp:36-80) of instance idx's current S:
+ * W (x*z), H (z*y), L (y), in double, host arrays (any may be NULL). After
+ * a solve the axis sums it needs are already in the workspace (the last
+ * pass accumulates them), so no pass over S runs; otherwise one sums pass. */
+bnmf_status bnmf_batch_fact(bnmf_batch* b, size_t idx, double eps, double* W, double* H, double* L);
 
 typedef struct bnmf_batch_info {
     size_t n_instances;
@@ -206,8 +219,7 @@ bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_
 
 /* Evaluates the device digamma forms on the GPU (kind 0: FP32 bulk psi,
  * 1: exact FP64, 2: util::psi_appr in FP64, src/util/util.hpp:440-514,
- * 3: FP32 psi(s+1) of the generic update pass at x = s+1, 4: FP32 psi of
- * the exact-z update pass (log2 units, times ln 2), x >= 1).
+ * 3: FP32 psi(s+1) of the update passes evaluated at x = s+1).
  * x and out are host arrays of n doubles. */
 bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n);
 
