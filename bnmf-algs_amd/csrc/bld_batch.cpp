@@ -268,7 +268,8 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         BNMF_HIP(hipMemcpy(b + L.off_S, S0, xyz * es_, hipMemcpyHostToDevice));
     }
 
-    // descriptors and work lists
+    // descriptors and work lists of the whole batch (the sums-only and
+    // scoring passes, bld_fact, tensor_sums)
     std::vector<BldInst> d_first, d_main;
     std::vector<PrepItem> prep;
     for (size_t n = 0; n < specs.size(); ++n) {
@@ -282,69 +283,48 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     }
     // one pass launch per kernel key, separately for the update pass and for
     // the sums-only / scoring passes
-    std::map<PassKey, std::vector<PassItem>> items_of[2];
-    for (int g = 0; g < 2; ++g) {
-        const int mode = g == 0 ? MODE_UPDATE : MODE_SUMS;
-        for (size_t n = 0; n < specs.size(); ++n) {
-            const InstLayout& L = layouts_[n];
-            if (L.elements() == 0) continue;
-            auto& v = items_of[g][pass_key(dtype, mode, L, nonneg[n])];
-            for (uint32_t ti = 0; ti < L.n_ti; ++ti)
-                for (uint32_t tj = 0; tj < L.n_tj; ++tj) v.push_back(PassItem{(uint32_t)n, ti, tj, 0});
-        }
+    std::map<PassKey, std::vector<PassItem>> sum_items;
+    std::map<PassKey, std::vector<uint32_t>> members;  // update groups: batch instance numbers
+    for (size_t n = 0; n < specs.size(); ++n) {
+        const InstLayout& L = layouts_[n];
+        if (L.elements() == 0) continue;
+        auto& v = sum_items[pass_key(dtype, MODE_SUMS, L, nonneg[n])];
+        for (uint32_t ti = 0; ti < L.n_ti; ++ti)
+            for (uint32_t tj = 0; tj < L.n_tj; ++tj) v.push_back(PassItem{(uint32_t)n, ti, tj, 0});
+        members[pass_key(dtype, MODE_UPDATE, L, nonneg[n])].push_back((uint32_t)n);
     }
-    size_t moff = 0;
-    auto mtake = [&](size_t nbytes) {
-        const size_t o = moff;
-        moff = align_up(moff + nbytes);
-        return o;
+    auto tile_work = [&](const InstLayout& L, uint32_t ti, uint32_t tj) {
+        const uint32_t rows = std::min(L.ti_rows, L.x - ti * L.ti_rows);
+        const uint32_t cols = std::min(kTileCols, L.y - tj * kTileCols);
+        return (size_t)rows * cols * L.z;
     };
-    off_desc_first_ = mtake(d_first.size() * sizeof(BldInst));
-    off_desc_ = mtake(d_main.size() * sizeof(BldInst));
-    off_prep_items_ = mtake(prep.size() * sizeof(PrepItem));
-    off_prep_count_ = mtake(sizeof(uint32_t));
-    n_prep_items_ = (uint32_t)prep.size();
-    for (int g = 0; g < 2; ++g) {
-        for (auto& kv : items_of[g]) {
-            Bucket bk;
-            bk.key = kv.first;
-            bk.zmax = 1;
-            bk.work = 0;
-            for (const PassItem& pi : kv.second) {
-                const InstLayout& L = layouts_[pi.inst];
-                const uint32_t rows = std::min(L.ti_rows, L.x - pi.ti * L.ti_rows);
-                const uint32_t cols = std::min(kTileCols, L.y - pi.tj * kTileCols);
-                bk.work += (size_t)rows * cols * L.z;
-                bk.zmax = std::max(bk.zmax, L.z);
-            }
-            bk.lane = 0;
-            bk.n_items = (uint32_t)kv.second.size();
-            bk.off_items = mtake(kv.second.size() * sizeof(PassItem));
-            bk.off_count = mtake(sizeof(uint32_t));
-            groups_[g].push_back(bk);
+    for (auto& kv : sum_items) {
+        Bucket bk;
+        bk.key = kv.first;
+        bk.zmax = 1;
+        bk.work = 0;
+        for (const PassItem& pi : kv.second) {
+            bk.work += tile_work(layouts_[pi.inst], pi.ti, pi.tj);
+            bk.zmax = std::max(bk.zmax, layouts_[pi.inst].z);
         }
+        bk.lane = 0;
+        bk.n_items = (uint32_t)kv.second.size();
+        groups_[1].push_back(bk);
     }
-    meta_ = DeviceBuffer(std::max<size_t>(moff, kAlign));
-    device_bytes_ += meta_.bytes;
-    char* m = meta_.as<char>();
-    if (!d_first.empty()) {
-        BNMF_HIP(hipMemcpy(m + off_desc_first_, d_first.data(), d_first.size() * sizeof(BldInst), hipMemcpyHostToDevice));
-        BNMF_HIP(hipMemcpy(m + off_desc_, d_main.data(), d_main.size() * sizeof(BldInst), hipMemcpyHostToDevice));
-    }
-    if (!prep.empty())
-        BNMF_HIP(hipMemcpy(m + off_prep_items_, prep.data(), prep.size() * sizeof(PrepItem), hipMemcpyHostToDevice));
-    BNMF_HIP(hipMemcpy(m + off_prep_count_, &n_prep_items_, sizeof(uint32_t), hipMemcpyHostToDevice));
-    for (int g = 0; g < 2; ++g) {
-        size_t bpos = 0;
-        for (auto& kv : items_of[g]) {
-            Bucket& bk = groups_[g][bpos++];
-            BNMF_HIP(hipMemcpy(m + bk.off_items, kv.second.data(), kv.second.size() * sizeof(PassItem), hipMemcpyHostToDevice));
-            BNMF_HIP(hipMemcpy(m + bk.off_count, &bk.n_items, sizeof(uint32_t), hipMemcpyHostToDevice));
+    for (auto& kv : members) {
+        Bucket bk;
+        bk.key = kv.first;
+        bk.zmax = 1;
+        bk.work = 0;
+        for (uint32_t n : kv.second) {
+            bk.work += layouts_[n].elements();
+            bk.zmax = std::max(bk.zmax, layouts_[n].z);
         }
+        bk.lane = 0;
+        groups_[0].push_back(bk);
     }
-    BNMF_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    // spread each pass's launches over up to four streams, largest work
-    // first onto the least loaded stream
+    // up to four streams; each group goes to the least loaded one, largest
+    // work first
     const int ns = (int)std::min<size_t>(4, std::max(groups_[0].size(), groups_[1].size()));
     if (ns > 1) {
         for (int g = 0; g < 2; ++g) {
@@ -359,6 +339,98 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
                 load[l] += gs[b].work;
             }
         }
+    }
+    // Each stream (lane) has its own descriptor array and prep work list
+    // over the instances of its update groups; their pass items use the
+    // lane's instance numbers.
+    lanes_.assign(std::max(ns, 1), Lane{});
+    std::vector<std::vector<uint32_t>> lane_members(lanes_.size());
+    std::vector<std::vector<PassItem>> upd_items(groups_[0].size());
+    {
+        size_t gi = 0;
+        for (auto& kv : members) {
+            Bucket& bk = groups_[0][gi];
+            std::vector<uint32_t>& lm = lane_members[bk.lane];
+            for (uint32_t n : kv.second) {
+                const uint32_t local = (uint32_t)lm.size();
+                lm.push_back(n);
+                const InstLayout& L = layouts_[n];
+                for (uint32_t ti = 0; ti < L.n_ti; ++ti)
+                    for (uint32_t tj = 0; tj < L.n_tj; ++tj) upd_items[gi].push_back(PassItem{local, ti, tj, 0});
+            }
+            bk.n_items = (uint32_t)upd_items[gi].size();
+            ++gi;
+        }
+    }
+    size_t moff = 0;
+    auto mtake = [&](size_t nbytes) {
+        const size_t o = moff;
+        moff = align_up(moff + nbytes);
+        return o;
+    };
+    off_desc_first_ = mtake(d_first.size() * sizeof(BldInst));
+    off_desc_ = mtake(d_main.size() * sizeof(BldInst));
+    off_prep_items_ = mtake(prep.size() * sizeof(PrepItem));
+    off_prep_count_ = mtake(sizeof(uint32_t));
+    n_prep_items_ = (uint32_t)prep.size();
+    for (int g = 0; g < 2; ++g)
+        for (Bucket& bk : groups_[g]) {
+            bk.off_items = mtake((size_t)bk.n_items * sizeof(PassItem));
+            bk.off_count = mtake(sizeof(uint32_t));
+        }
+    for (size_t l = 0; l < lanes_.size(); ++l) {
+        Lane& ln = lanes_[l];
+        ln.n_inst = (uint32_t)lane_members[l].size();
+        for (uint32_t n : lane_members[l]) ln.n_prep += layouts_[n].n_cj + layouts_[n].nA;
+        ln.off_desc = mtake(ln.n_inst * sizeof(BldInst));
+        ln.off_desc_first = mtake(ln.n_inst * sizeof(BldInst));
+        ln.off_prep = mtake((size_t)ln.n_prep * sizeof(PrepItem));
+        ln.off_prep_count = mtake(sizeof(uint32_t));
+    }
+    meta_ = DeviceBuffer(std::max<size_t>(moff, kAlign));
+    device_bytes_ += meta_.bytes;
+    char* m = meta_.as<char>();
+    if (!d_first.empty()) {
+        BNMF_HIP(hipMemcpy(m + off_desc_first_, d_first.data(), d_first.size() * sizeof(BldInst), hipMemcpyHostToDevice));
+        BNMF_HIP(hipMemcpy(m + off_desc_, d_main.data(), d_main.size() * sizeof(BldInst), hipMemcpyHostToDevice));
+    }
+    if (!prep.empty())
+        BNMF_HIP(hipMemcpy(m + off_prep_items_, prep.data(), prep.size() * sizeof(PrepItem), hipMemcpyHostToDevice));
+    BNMF_HIP(hipMemcpy(m + off_prep_count_, &n_prep_items_, sizeof(uint32_t), hipMemcpyHostToDevice));
+    {
+        size_t gi = 0;
+        for (auto& kv : sum_items) {
+            Bucket& bk = groups_[1][gi++];
+            BNMF_HIP(hipMemcpy(m + bk.off_items, kv.second.data(), kv.second.size() * sizeof(PassItem), hipMemcpyHostToDevice));
+            BNMF_HIP(hipMemcpy(m + bk.off_count, &bk.n_items, sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
+        for (gi = 0; gi < groups_[0].size(); ++gi) {
+            Bucket& bk = groups_[0][gi];
+            BNMF_HIP(hipMemcpy(m + bk.off_items, upd_items[gi].data(), upd_items[gi].size() * sizeof(PassItem), hipMemcpyHostToDevice));
+            BNMF_HIP(hipMemcpy(m + bk.off_count, &bk.n_items, sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
+    }
+    for (size_t l = 0; l < lanes_.size(); ++l) {
+        const Lane& ln = lanes_[l];
+        std::vector<BldInst> dm, df;
+        std::vector<PrepItem> lp;
+        for (uint32_t loc = 0; loc < ln.n_inst; ++loc) {
+            const uint32_t n = lane_members[l][loc];
+            dm.push_back(d_main[n]);
+            df.push_back(d_first[n]);
+            for (uint32_t c = 0; c < layouts_[n].n_cj; ++c) lp.push_back(PrepItem{loc, 0, c, 0});
+            for (uint32_t c = 0; c < layouts_[n].nA; ++c) lp.push_back(PrepItem{loc, 1, c, 0});
+        }
+        if (ln.n_inst) {
+            BNMF_HIP(hipMemcpy(m + ln.off_desc, dm.data(), dm.size() * sizeof(BldInst), hipMemcpyHostToDevice));
+            BNMF_HIP(hipMemcpy(m + ln.off_desc_first, df.data(), df.size() * sizeof(BldInst), hipMemcpyHostToDevice));
+        }
+        if (ln.n_prep)
+            BNMF_HIP(hipMemcpy(m + ln.off_prep, lp.data(), lp.size() * sizeof(PrepItem), hipMemcpyHostToDevice));
+        BNMF_HIP(hipMemcpy(m + ln.off_prep_count, &ln.n_prep, sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    BNMF_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (ns > 1) {
         side_.resize(ns);
         fork_ev_.resize(ns);
         join_ev_.resize(ns);
@@ -392,7 +464,12 @@ void Batch::enqueue_pass(int mode, bool first) {
     for (const Bucket& bk : groups_of(mode)) {
         const dim3 grid(bk.n_items);
         hipStream_t st = fork ? side_[bk.lane] : stream_;
-        BNMF_HIP(launch_pass_key(bk.key, dtype_, psi_, mode, grid, st, d,
+        const BldInst* dg =
+            mode == MODE_UPDATE
+                ? reinterpret_cast<const BldInst*>(
+                      m + (first ? lanes_[bk.lane].off_desc_first : lanes_[bk.lane].off_desc))
+                : d;
+        BNMF_HIP(launch_pass_key(bk.key, dtype_, psi_, mode, grid, st, dg,
                                  reinterpret_cast<const PassItem*>(m + bk.off_items),
                                  reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
     }
@@ -404,14 +481,44 @@ void Batch::enqueue_pass(int mode, bool first) {
     }
 }
 
-void Batch::enqueue_iteration(bool first) {
-    if (n_prep_items_ == 0) return;
+void Batch::enqueue_iterations(int iters, bool first, bool with_prep) {
+    if (n_prep_items_ == 0 || iters <= 0) return;
     char* m = meta_.as<char>();
-    const BldInst* d = reinterpret_cast<const BldInst*>(m + off_desc_);
-    BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(n_prep_items_), stream_, d,
-                         reinterpret_cast<const PrepItem*>(m + off_prep_items_),
-                         reinterpret_cast<const uint32_t*>(m + off_prep_count_), (uint32_t)layouts_.size()));
-    enqueue_pass(MODE_UPDATE, first);
+    const bool fork = side_.size() > 1;
+    if (fork) {
+        BNMF_HIP(hipEventRecord(fork_ev_[0], stream_));
+        for (size_t l = 1; l < side_.size(); ++l) BNMF_HIP(hipStreamWaitEvent(side_[l], fork_ev_[0], 0));
+    }
+    // Instances of different streams never interact, so each stream runs
+    // its own iterations back to back (prep and psi(A) of its instances,
+    // then its groups' passes): one stream's prep overlaps the others'
+    // passes and no stream waits for the others between iterations.
+    for (int t = 0; t < iters; ++t) {
+        for (size_t l = 0; l < lanes_.size(); ++l) {
+            const Lane& ln = lanes_[l];
+            if (ln.n_inst == 0) continue;
+            hipStream_t st = fork ? side_[l] : stream_;
+            if (with_prep)
+                BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(ln.n_prep), st,
+                                     reinterpret_cast<const BldInst*>(m + ln.off_desc),
+                                     reinterpret_cast<const PrepItem*>(m + ln.off_prep),
+                                     reinterpret_cast<const uint32_t*>(m + ln.off_prep_count), ln.n_inst));
+            const BldInst* dp =
+                reinterpret_cast<const BldInst*>(m + ((first && t == 0) ? ln.off_desc_first : ln.off_desc));
+            for (const Bucket& bk : groups_[0]) {
+                if ((size_t)bk.lane != l) continue;
+                BNMF_HIP(launch_pass_key(bk.key, dtype_, psi_, MODE_UPDATE, dim3(bk.n_items), st, dp,
+                                         reinterpret_cast<const PassItem*>(m + bk.off_items),
+                                         reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
+            }
+        }
+    }
+    if (fork) {
+        for (size_t l = 1; l < side_.size(); ++l) {
+            BNMF_HIP(hipEventRecord(join_ev_[l], side_[l]));
+            BNMF_HIP(hipStreamWaitEvent(stream_, join_ev_[l], 0));
+        }
+    }
 }
 
 void Batch::capture_graphs() {
@@ -420,7 +527,7 @@ void Batch::capture_graphs() {
         hipGraph_t g = nullptr;
         BNMF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         try {
-            for (int t = 0; t < iters; ++t) enqueue_iteration(false);
+            enqueue_iterations(iters, false);
         } catch (...) {
             (void)hipStreamEndCapture(stream_, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -509,19 +616,14 @@ void Batch::time_pass(int reps, double* pass_ms, double* iter_ms) {
     }
     partials_valid_ = false;
     // Two graphs, launched the way solves launch them: reps update passes
-    // (every pass launch of an iteration, forked over the side streams),
-    // and reps whole iterations (prep + pass). Each is replayed once to
-    // warm up, then timed with events on the batch stream.
+    // (every pass launch of an iteration, each group's on its stream), and
+    // reps whole iterations (prep + pass). Each is replayed once to warm up,
+    // then timed with events on the batch stream.
     auto capture = [&](bool with_prep, hipGraphExec_t* out) {
         hipGraph_t g = nullptr;
         BNMF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         try {
-            for (int r = 0; r < reps; ++r) {
-                if (with_prep)
-                    enqueue_iteration(false);
-                else
-                    enqueue_pass(MODE_UPDATE, false);
-            }
+            enqueue_iterations(reps, false, with_prep);
         } catch (...) {
             (void)hipStreamEndCapture(stream_, &g);
             if (g) (void)hipGraphDestroy(g);

@@ -175,7 +175,11 @@ class Batch {
         size_t off_items;  // in meta_
         size_t off_count;
     };
-    void enqueue_iteration(bool first);
+    void enqueue_iteration(bool first) { enqueue_iterations(1, first); }
+    // `iters` iterations; each update group runs prep, psi(A) and its pass
+    // on its own stream, iteration after iteration, with one fork and join
+    // (with_prep = false: the update passes alone, for timing)
+    void enqueue_iterations(int iters, bool first, bool with_prep = true);
     void enqueue_pass(int mode, bool first);
     // pass launches of the update pass (groups_[0]) and of the sums-only and
     // scoring passes (groups_[1])
@@ -193,6 +197,13 @@ class Batch {
     size_t off_desc_first_ = 0, off_desc_ = 0, off_prep_items_ = 0, off_prep_count_ = 0;
     uint32_t n_prep_items_ = 0;
     std::vector<Bucket> groups_[2];
+    // per stream: descriptor arrays of the instances of its update groups
+    // (the groups' pass items index them), its prep work list
+    struct Lane {
+        size_t off_desc = 0, off_desc_first = 0, off_prep = 0, off_prep_count = 0;
+        uint32_t n_prep = 0, n_inst = 0;
+    };
+    std::vector<Lane> lanes_;
     hipStream_t stream_ = nullptr;
     // z-bucket passes of one iteration are independent: they run on up to
     // four streams forked from and joined back to stream_ (graph branches
