@@ -278,12 +278,13 @@ def main():
     elif not args.no_autotune:
         # tile heights per shape class, chosen by the autotuner on this
         # workload (untimed setup; the table applies to batches created after)
-        log("[bench] autotuning tile heights on the batch's shape classes")
+        log("[bench] autotuning tile heights on the batch (whole-iteration timings)")
         t0 = time.perf_counter()
-        res = hip.autotune(np.float32, instances=c3f, candidates=(16, 32, 64, 128), reps=5,
-                           trials=2, log=log)
-        tuned = {"seconds": round(time.perf_counter() - t0, 1), "classes": len(res),
-                 "rows": {f"z{k[0]}f{k[1]}": v["rows"] for k, v in sorted(res.items())}}
+        res = hip.autotune_workload(np.float32, c3f, candidates=(16, 32, 64), reps=10,
+                                    trials=2, refine=True, log=log)
+        tuned = {"seconds": round(time.perf_counter() - t0, 1), "classes": len(res["rows"]),
+                 "evaluations": res["evaluations"], "iter_us": round(res["iter_ms"] * 1e3, 1),
+                 "rows": {f"z{k[0]}f{k[1]}": v for k, v in sorted(res["rows"].items())}}
         if args.save_tune:
             hip.tune_save(args.save_tune)
     b32 = hip.Batch(c3f, dtype=np.float32, use_psi_appr=True)

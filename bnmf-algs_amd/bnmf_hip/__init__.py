@@ -564,6 +564,63 @@ def _tune_class(z: int, fibers: int):
     return zc, fc
 
 
+def autotune_workload(dtype, instances: Sequence[Instance], candidates=(16, 32, 64),
+                      reps: int = 10, trials: int = 2, refine: bool = True,
+                      use_psi_appr: bool = True, log=None) -> dict:
+    """Chooses tile heights for a workload solved as one batch, timing whole
+    iterations of the whole batch (graph-launched, as solves run them), so
+    the effects of running all shape classes together are measured: first
+    one height for every class (best of `candidates`), then, if `refine`,
+    each class (largest first) tries the other candidates with the rest
+    fixed and keeps a change that makes the iteration at least 1 % faster.
+    Fills the tile table; returns {"rows": {(zc, fc): rows}, "iter_ms": t,
+    "evaluations": n}."""
+    dt = _np_dtype(dtype)
+    groups = {}
+    for inst in instances:
+        x, y = np.shape(inst.X)
+        if x * y * inst.z:
+            groups.setdefault(_tune_class(inst.z, x * y), []).append(inst)
+    reps_of = {k: (v[0].z, np.shape(v[0].X)[0] * np.shape(v[0].X)[1]) for k, v in groups.items()}
+    evals = 0
+
+    def measure(choice):
+        nonlocal evals
+        for k, rows in choice.items():
+            tune_set(dt, reps_of[k][0], reps_of[k][1], rows)
+        with Batch(instances, dtype=dt, use_psi_appr=use_psi_appr) as b:
+            b.solve(2)
+            t = min(b.time_pass(reps)[1] for _ in range(trials))
+        evals += 1
+        return t
+
+    best_t, best = None, None
+    for c in candidates:
+        choice = {k: c for k in groups}
+        t = measure(choice)
+        if log:
+            log(f"[autotune] all classes {c} rows: {t * 1e3:.1f} us per iteration")
+        if best_t is None or t < best_t:
+            best_t, best = t, choice
+    if refine:
+        order = sorted(groups, key=lambda k: -sum(np.size(i.X) * i.z for i in groups[k]))
+        for k in order:
+            for c in candidates:
+                if c == best[k]:
+                    continue
+                trial = dict(best)
+                trial[k] = c
+                t = measure(trial)
+                if t < best_t * 0.99:
+                    if log:
+                        log(f"[autotune] class z{k[0]} f{k[1]}: {best[k]} -> {c} rows, "
+                            f"{best_t * 1e3:.1f} -> {t * 1e3:.1f} us")
+                    best_t, best = t, trial
+    for k, rows in best.items():
+        tune_set(dt, reps_of[k][0], reps_of[k][1], rows)
+    return {"rows": best, "iter_ms": best_t, "evaluations": evals}
+
+
 def autotune(dtype=np.float32, instances: Optional[Sequence[Instance]] = None,
              z_classes=TUNE_Z_CLASSES, fiber_classes=TUNE_FIBER_CLASSES,
              candidates=(16, 32, 64, 128, 256), reps: int = 10, trials: int = 3,
