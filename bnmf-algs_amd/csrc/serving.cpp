@@ -238,10 +238,18 @@ class Server {
         size_t off_items, off_count, work;
     };
     // Device-side work lists for a set of instances.
+    // Update work sets also keep, per stream, a descriptor array and prep
+    // list over the instances of that stream's launches (their pass items
+    // index it), so each stream can iterate on its own.
+    struct LaneWS {
+        size_t off_desc = 0, off_prep = 0, off_prep_count = 0;
+        uint32_t n_prep = 0, n_inst = 0;
+    };
     struct WorkSet {
         std::vector<Launch> launches;
         size_t off_desc = 0, off_prep = 0, off_prep_count = 0;
         uint32_t n_prep = 0, n_inst = 0;
+        LaneWS lanes[4];
     };
 
     void loop() {
@@ -323,7 +331,7 @@ class Server {
         };
         std::vector<BldInst> desc;
         std::vector<PrepItem> prep;
-        std::map<PassKey, std::vector<PassItem>> groups;
+        std::map<PassKey, std::vector<uint32_t>> members;
         std::map<PassKey, uint32_t> zmax;
         std::map<PassKey, size_t> work;
         for (size_t n = 0; n < set.size(); ++n) {
@@ -334,8 +342,7 @@ class Server {
             if (L.elements() == 0) continue;
             const PassKey key = pass_key(dtype_, mode, L, r.nonneg);
             zmax[key] = std::max(zmax[key], L.z);
-            for (uint32_t ti = 0; ti < L.n_ti; ++ti)
-                for (uint32_t tj = 0; tj < L.n_tj; ++tj) groups[key].push_back(PassItem{(uint32_t)n, ti, tj, 0});
+            members[key].push_back((uint32_t)n);
             work[key] += L.elements();
             for (uint32_t c = 0; c < L.n_cj; ++c) prep.push_back(PrepItem{(uint32_t)n, 0, c, 0});
             for (uint32_t c = 0; c < L.nA; ++c) prep.push_back(PrepItem{(uint32_t)n, 1, c, 0});
@@ -351,22 +358,57 @@ class Server {
         ws.launches.clear();
         std::vector<size_t> load(4, 0);
         std::vector<std::pair<size_t, PassKey>> order;
-        for (auto& g : groups) order.push_back({work[g.first], g.first});
+        for (auto& g : members) order.push_back({work[g.first], g.first});
         std::stable_sort(order.begin(), order.end(), [](auto& a, auto& b) { return a.first > b.first; });
+        // update sets: pass items number instances within their stream
+        const bool per_lane = mode == MODE_UPDATE;
+        std::vector<uint32_t> lane_members[4];
         for (auto& o : order) {
-            auto& items = groups[o.second];
             Launch l;
             l.key = o.second;
             l.zmax = zmax[o.second];
-            l.n = (uint32_t)items.size();
             l.work = o.first;
             l.lane = (int)(std::min_element(load.begin(), load.end()) - load.begin());
             load[l.lane] += l.work;
+            std::vector<PassItem> items;
+            for (uint32_t n : members[o.second]) {
+                uint32_t idx = n;
+                if (per_lane) {
+                    idx = (uint32_t)lane_members[l.lane].size();
+                    lane_members[l.lane].push_back(n);
+                }
+                const InstLayout& L = set[n]->L;
+                for (uint32_t ti = 0; ti < L.n_ti; ++ti)
+                    for (uint32_t tj = 0; tj < L.n_tj; ++tj) items.push_back(PassItem{idx, ti, tj, 0});
+            }
+            l.n = (uint32_t)items.size();
             l.off_items = take(items.size() * sizeof(PassItem));
             std::memcpy(host.data() + l.off_items, items.data(), items.size() * sizeof(PassItem));
             l.off_count = take(4);
             std::memcpy(host.data() + l.off_count, &l.n, 4);
             ws.launches.push_back(l);
+        }
+        if (per_lane) {
+            for (int ln = 0; ln < 4; ++ln) {
+                LaneWS& lw = ws.lanes[ln];
+                std::vector<BldInst> ld;
+                std::vector<PrepItem> lp;
+                for (uint32_t loc = 0; loc < (uint32_t)lane_members[ln].size(); ++loc) {
+                    const uint32_t n = lane_members[ln][loc];
+                    ld.push_back(desc[n]);
+                    const InstLayout& L = set[n]->L;
+                    for (uint32_t c = 0; c < L.n_cj; ++c) lp.push_back(PrepItem{loc, 0, c, 0});
+                    for (uint32_t c = 0; c < L.nA; ++c) lp.push_back(PrepItem{loc, 1, c, 0});
+                }
+                lw.n_inst = (uint32_t)ld.size();
+                lw.n_prep = (uint32_t)lp.size();
+                lw.off_desc = take(ld.size() * sizeof(BldInst));
+                if (!ld.empty()) std::memcpy(host.data() + lw.off_desc, ld.data(), ld.size() * sizeof(BldInst));
+                lw.off_prep = take(lp.size() * sizeof(PrepItem));
+                if (!lp.empty()) std::memcpy(host.data() + lw.off_prep, lp.data(), lp.size() * sizeof(PrepItem));
+                lw.off_prep_count = take(4);
+                std::memcpy(host.data() + lw.off_prep_count, &lw.n_prep, 4);
+            }
         }
         return moff;
     }
@@ -399,6 +441,43 @@ class Server {
         }
     }
 
+    // iters iterations of an update work set: each stream preps and
+    // updates its own instances, iteration after iteration, with one fork
+    // and one join (instances on different streams never interact)
+    void iterate(const WorkSet& ws, size_t iters) {
+        char* m = meta_.as<char>();
+        int used = 0;
+        for (int l = 0; l < 4; ++l) used += ws.lanes[l].n_inst > 0;
+        const bool fork = used > 1;
+        if (fork) {
+            BNMF_HIP(hipEventRecord(fork_[0], stream_));
+            for (int l = 1; l < 4; ++l) BNMF_HIP(hipStreamWaitEvent(side_[l], fork_[0], 0));
+        }
+        for (size_t t = 0; t < iters; ++t) {
+            for (int l = 0; l < 4; ++l) {
+                const LaneWS& lw = ws.lanes[l];
+                if (lw.n_inst == 0) continue;
+                hipStream_t st = fork ? side_[l] : stream_;
+                const BldInst* d = reinterpret_cast<const BldInst*>(m + lw.off_desc);
+                BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(lw.n_prep), st, d,
+                                     reinterpret_cast<const PrepItem*>(m + lw.off_prep),
+                                     reinterpret_cast<const uint32_t*>(m + lw.off_prep_count), lw.n_inst));
+                for (const Launch& ln : ws.launches) {
+                    if (ln.lane != l) continue;
+                    BNMF_HIP(launch_pass_key(ln.key, dtype_, psi_, MODE_UPDATE, dim3(ln.n), st, d,
+                                             reinterpret_cast<const PassItem*>(m + ln.off_items),
+                                             reinterpret_cast<const uint32_t*>(m + ln.off_count), ln.zmax));
+                }
+            }
+        }
+        if (fork) {
+            for (int l = 1; l < 4; ++l) {
+                BNMF_HIP(hipEventRecord(join_[l], side_[l]));
+                BNMF_HIP(hipStreamWaitEvent(stream_, join_[l], 0));
+            }
+        }
+    }
+
     void step(const std::vector<std::shared_ptr<Request>>& act) {
         const double t0 = now_s();
         std::vector<Request*> all, fresh;
@@ -416,14 +495,7 @@ class Server {
         BNMF_HIP(hipMemcpyAsync(meta_.ptr, host.data(), end, hipMemcpyHostToDevice, stream_));
         char* m = meta_.as<char>();
         if (!fresh.empty()) passes(ws_fresh, MODE_SUMS);
-        for (size_t t = 0; t < k_iters; ++t) {
-            if (ws_all.n_prep)
-                BNMF_HIP(launch_prep(dtype_, psi_, false, dim3(ws_all.n_prep), stream_,
-                                     reinterpret_cast<const BldInst*>(m + ws_all.off_desc),
-                                     reinterpret_cast<const PrepItem*>(m + ws_all.off_prep),
-                                     reinterpret_cast<const uint32_t*>(m + ws_all.off_prep_count), ws_all.n_inst));
-            passes(ws_all, MODE_UPDATE);
-        }
+        iterate(ws_all, k_iters);
         // requests that finish in this step, scored if asked
         std::vector<Request*> fin, score;
         for (Request* r : all) {
