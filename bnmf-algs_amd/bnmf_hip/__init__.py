@@ -85,6 +85,11 @@ def lib():
         L.bnmf_bld_mult_f64.restype = st
         L.bnmf_bld_mult_f32.argtypes = L.bnmf_bld_mult_f64.argtypes
         L.bnmf_bld_mult_f32.restype = st
+        L.bnmf_bld_add_f64.argtypes = [_VP, _SZ, _SZ, _SZ, _D, _D, _SZ, ctypes.c_double, _VP,
+                                       ctypes.c_uint64, _VP]
+        L.bnmf_bld_add_f64.restype = st
+        L.bnmf_bld_add_f32.argtypes = L.bnmf_bld_add_f64.argtypes
+        L.bnmf_bld_add_f32.restype = st
         L.bnmf_log_marginal_S.argtypes = [ctypes.c_int, _VP, _SZ, _SZ, _SZ, _D, _D,
                                           ctypes.c_double, ctypes.c_double, _D]
         L.bnmf_log_marginal_S.restype = st
@@ -94,6 +99,10 @@ def lib():
                                         ctypes.POINTER(bnmf_bld_desc), _SZ, ctypes.c_int,
                                         ctypes.c_double]
         L.bnmf_batch_create.restype = st
+        L.bnmf_batch_create_algo.argtypes = [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(bnmf_bld_desc), _SZ, ctypes.c_int,
+                                             ctypes.c_double]
+        L.bnmf_batch_create_algo.restype = st
         for name in ("bnmf_batch_solve", "bnmf_batch_solve_async"):
             getattr(L, name).argtypes = [_VP, _SZ]
             getattr(L, name).restype = st
@@ -251,6 +260,33 @@ def bld_mult(X, z: int, model_params: Params, max_iter: int = 1000,
 # The reference's GPU driver has the same signature; both run on the GPU here.
 bld_mult_cuda = bld_mult
 
+ALGO_MULT = 0
+ALGO_ADD = 1
+
+
+def bld_add(X, z: int, model_params: Params, max_iter: int = 1000, eps: float = 1e-50, *,
+            S0=None, seed: int = 0, dtype=np.float64) -> np.ndarray:
+    """bld::bld_add (src/bld/bld_add.hpp:69-184): S (x, y, z) with S_ij+ = X
+    after max_iter additive gradient-ascent steps on log S, on the GPU.
+    Every digamma is psi_appr, as in the reference. dtype float64 or
+    float32; S0 overrides the seeded initial tensor."""
+    dt = _np_dtype(dtype)
+    X = np.ascontiguousarray(X, dtype=dt)
+    _check_bld_params(X, z, model_params)
+    x, y = X.shape
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    S0a = None
+    if S0 is not None:
+        S0a = np.ascontiguousarray(S0, dtype=dt)
+        if S0a.shape != (x, y, z):
+            raise ValueError("S0 must have shape (x, y, z)")
+    S = np.empty((x, y, z), dtype=dt)
+    fn = lib().bnmf_bld_add_f32 if dt == np.float32 else lib().bnmf_bld_add_f64
+    _check(fn(_ptr(X), x, y, z, _dptr(alpha), _dptr(beta), int(max_iter), float(eps), _ptr(S0a),
+              int(seed) & (2**64 - 1), _ptr(S)), "bld_add")
+    return S
+
 
 def log_marginal_S(S, model_params: Params) -> float:
     """alloc_model::log_marginal_S in FP64 on the GPU."""
@@ -381,13 +417,14 @@ class Batch:
     hipGraph). Inputs are uploaded once; S stays resident in HBM."""
 
     def __init__(self, instances: Sequence[Instance], dtype=np.float32,
-                 use_psi_appr: bool = True, eps: float = 1e-50):
+                 use_psi_appr: bool = True, eps: float = 1e-50, algo: int = ALGO_MULT):
         self.dt = _np_dtype(dtype)
         self.shapes = [(np.shape(i.X)[0], np.shape(i.X)[1], i.z) for i in instances]
         arr, keep = _descs(instances, self.dt)
         h = _VP()
-        _check(lib().bnmf_batch_create(ctypes.byref(h), _code(self.dt), arr, len(instances),
-                                       int(bool(use_psi_appr)), float(eps)), "batch_create")
+        _check(lib().bnmf_batch_create_algo(ctypes.byref(h), _code(self.dt), int(algo), arr,
+                                            len(instances), int(bool(use_psi_appr)), float(eps)),
+               "batch_create")
         del keep
         self._h = h
 

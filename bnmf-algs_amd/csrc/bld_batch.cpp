@@ -191,7 +191,7 @@ PassKey pass_key(int dtype, int mode, const InstLayout& L, bool nonneg) {
         return k;
     }
     k.zb = (int)L.zb;
-    k.rw = mode == MODE_UPDATE ? (int)L.rw : 1;
+    k.rw = (mode == MODE_UPDATE || mode == MODE_ADD) ? (int)L.rw : 1;
     return k;
 }
 
@@ -206,8 +206,14 @@ hipError_t launch_pass_key(const PassKey& k, int dtype, int psi, int mode, dim3 
 }
 
 // ------------------------------------------------------------------ batch
-Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& specs)
-    : dtype_(dtype), psi_(psi_mode), eps_(eps), es_(dtype == BNMF_F32 ? 4 : 8), specs_(specs) {
+Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& specs, int algo)
+    : dtype_(dtype), psi_(psi_mode), algo_(algo), eps_(eps), es_(dtype == BNMF_F32 ? 4 : 8), specs_(specs) {
+    BNMF_REQUIRE(algo == BNMF_ALGO_MULT || algo == BNMF_ALGO_ADD, "algo must be BNMF_ALGO_MULT or BNMF_ALGO_ADD");
+    // bld_add evaluates every digamma with psi_appr (src/bld/bld_add.hpp:137-146)
+    if (algo == BNMF_ALGO_ADD) {
+        step_mode_ = MODE_ADD;
+        if (dtype == BNMF_F64) psi_ = PSIM_F64_APPR;
+    }
     ensure_runtime();
     int dev = 0;
     BNMF_HIP(hipGetDevice(&dev));
@@ -291,7 +297,7 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         auto& v = sum_items[pass_key(dtype, MODE_SUMS, L, nonneg[n])];
         for (uint32_t ti = 0; ti < L.n_ti; ++ti)
             for (uint32_t tj = 0; tj < L.n_tj; ++tj) v.push_back(PassItem{(uint32_t)n, ti, tj, 0});
-        members[pass_key(dtype, MODE_UPDATE, L, nonneg[n])].push_back((uint32_t)n);
+        members[pass_key(dtype, step_mode_, L, nonneg[n])].push_back((uint32_t)n);
     }
     auto tile_work = [&](const InstLayout& L, uint32_t ti, uint32_t tj) {
         const uint32_t rows = std::min(L.ti_rows, L.x - ti * L.ti_rows);
@@ -370,6 +376,7 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     };
     off_desc_first_ = mtake(d_first.size() * sizeof(BldInst));
     off_desc_ = mtake(d_main.size() * sizeof(BldInst));
+    off_iter_ = mtake(specs.size() * sizeof(uint32_t));
     off_prep_items_ = mtake(prep.size() * sizeof(PrepItem));
     off_prep_count_ = mtake(sizeof(uint32_t));
     n_prep_items_ = (uint32_t)prep.size();
@@ -390,6 +397,14 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     meta_ = DeviceBuffer(std::max<size_t>(moff, kAlign));
     device_bytes_ += meta_.bytes;
     char* m = meta_.as<char>();
+    if (algo_ == BNMF_ALGO_ADD) {
+        for (size_t n = 0; n < specs.size(); ++n) {
+            uint32_t* ctr = reinterpret_cast<uint32_t*>(m + off_iter_) + n;
+            d_first[n].iter = ctr;
+            d_main[n].iter = ctr;
+        }
+        BNMF_HIP(hipMemset(m + off_iter_, 0, specs.size() * sizeof(uint32_t)));
+    }
     if (!d_first.empty()) {
         BNMF_HIP(hipMemcpy(m + off_desc_first_, d_first.data(), d_first.size() * sizeof(BldInst), hipMemcpyHostToDevice));
         BNMF_HIP(hipMemcpy(m + off_desc_, d_main.data(), d_main.size() * sizeof(BldInst), hipMemcpyHostToDevice));
@@ -465,7 +480,7 @@ void Batch::enqueue_pass(int mode, bool first) {
         const dim3 grid(bk.n_items);
         hipStream_t st = fork ? side_[bk.lane] : stream_;
         const BldInst* dg =
-            mode == MODE_UPDATE
+            (mode == MODE_UPDATE || mode == MODE_ADD)
                 ? reinterpret_cast<const BldInst*>(
                       m + (first ? lanes_[bk.lane].off_desc_first : lanes_[bk.lane].off_desc))
                 : d;
@@ -507,7 +522,7 @@ void Batch::enqueue_iterations(int iters, bool first, bool with_prep) {
                 reinterpret_cast<const BldInst*>(m + ((first && t == 0) ? ln.off_desc_first : ln.off_desc));
             for (const Bucket& bk : groups_[0]) {
                 if ((size_t)bk.lane != l) continue;
-                BNMF_HIP(launch_pass_key(bk.key, dtype_, psi_, MODE_UPDATE, dim3(bk.n_items), st, dp,
+                BNMF_HIP(launch_pass_key(bk.key, dtype_, psi_, step_mode_, dim3(bk.n_items), st, dp,
                                          reinterpret_cast<const PassItem*>(m + bk.off_items),
                                          reinterpret_cast<const uint32_t*>(m + bk.off_count), bk.zmax));
             }
@@ -557,6 +572,9 @@ void Batch::solve_async(size_t iters) {
         return;
     }
     partials_valid_ = false;
+    // the additive step size restarts with the solve
+    if (algo_ == BNMF_ALGO_ADD)
+        BNMF_HIP(hipMemsetAsync(meta_.as<char>() + off_iter_, 0, specs_.size() * sizeof(uint32_t), stream_));
     // partial axis sums of S0, then the first iteration reading S0
     enqueue_pass(MODE_SUMS, true);
     enqueue_iteration(true);

@@ -84,12 +84,16 @@ __global__ void __launch_bounds__(64 * WAVES)
         const double eps = I->eps;
         const bool fast_psi = I->nonneg != 0u;
 
-        if constexpr (MODE == PASS_UPDATE) {
+        if constexpr (pass_steps(MODE)) {
             const BNMF_GLOBAL T* __restrict__ psib_g = gconst<T>(I->psib) + (size_t)j0 * z;
             for (uint32_t e = tid; e < n_el; e += NT) psib_l[e] = psib_g[e];
             if (tid < z) psiA_l[tid] = gconst<T>(I->psiA)[tid];
             __syncthreads();
         }
+        // additive step size eta = 0.1 / (iteration + 1)^0.55 (bld_add.hpp:101-103);
+        // *iter counts the iterations started, this one included
+        double eta = 0.0;
+        if constexpr (MODE == PASS_ADD) eta = 0.1 / pow((double)*I->iter, 0.55);
 
         ACC acc[ZB];
 #pragma unroll
@@ -255,6 +259,100 @@ __global__ void __launch_bounds__(64 * WAVES)
                         }
                     }
                 }
+            } else if constexpr (MODE == PASS_ADD) {
+                // bld::bld_add's iteration (src/bld/bld_add.hpp:137-181) per
+                // fiber; the LDS row holds psi(alpha_eph(i,k)) here
+                const BNMF_GLOBAL T* __restrict__ psia_g = gconst<T>(I->psia) + (size_t)i * z;
+                if (lane < z) gm_l[lane] = psia_g[lane];
+                T xv = T(0);
+                if (lane < nj) xv = gconst<T>(I->X)[(size_t)i * y + j0 + lane];
+                wave_sync();
+                if (lane < nj) {
+                    T* const fb = raw + cur_dl + lane * z;
+                    const T* const pbb = psib_l + lane * z;
+                    T gz[ZB];  // grad_S, then exp(Z)
+                    T smult = T(0);
+                    // grad_S = psi(beta_eph) - psi(S+1) - psi(A) + psi(alpha_eph)
+                    // (:137-146) and S_mult = sum_k S grad_S (:148-155)
+#pragma unroll
+                    for (int m = 0; m < NCH; ++m) {
+                        if ((uint32_t)m < nchunk) {
+                            const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                            const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                            const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                            const VR pA = *reinterpret_cast<const VR*>(psiA_l + mm);
+                            const VR pa = *reinterpret_cast<const VR*>(gm_l + mm);
+#pragma unroll
+                            for (int v = 0; v < RW; ++v) {
+                                const T sk = vget<T, RW>(sv, v);
+                                T ps;
+                                if constexpr (sizeof(T) == 8)
+                                    ps = psi_sp1<PSI, T>(sk);
+                                else
+                                    ps = fast_psi ? psi_ge1_f32(sk + 1.0f) : psi_sp1<PSI_F32, float>(sk);
+                                const T g = ((vget<T, RW>(pb, v) - ps) - vget<T, RW>(pA, v)) + vget<T, RW>(pa, v);
+                                gz[m * RW + v] = g;
+                                smult += sk * g;
+                            }
+                        }
+                    }
+                    // Z = log S + eps + eta S (X grad_S - S_mult) (:150,158-166),
+                    // exp(Z) and its L1 norm along k with util::normalize's
+                    // default 1e-50 guard (:167-173, util.hpp:321-383)
+                    T norm = T(0);
+#pragma unroll
+                    for (int m = 0; m < NCH; ++m) {
+                        if ((uint32_t)m < nchunk) {
+                            const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                            const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+#pragma unroll
+                            for (int v = 0; v < RW; ++v) {
+                                const T sk = vget<T, RW>(sv, v);
+                                const T d = xv * gz[m * RW + v] - smult;
+                                T e;
+                                if constexpr (sizeof(T) == 8) {
+                                    T zk = log(sk) + eps;
+                                    zk += eta * sk * d;
+                                    e = exp(zk);
+                                } else {
+                                    // exp(log s + t) = s exp(t): one rounding of
+                                    // s instead of log's (0 stays 0)
+                                    e = sk * __expf(static_cast<float>(eta) * sk * d);
+                                }
+                                gz[m * RW + v] = e;
+                                norm += e;
+                            }
+                        }
+                    }
+                    // S = X exp(Z) / (|exp(Z)|_1 + 1e-50) (:175-181). In FP32
+                    // the guard underflows to 0: an all-zero row (X = 0)
+                    // is taken explicitly
+                    T scale;
+                    if constexpr (sizeof(T) == 8) {
+                        norm += 1e-50;
+                        scale = T(1);
+                    } else {
+                        scale = norm > 0.0f ? xv / norm : 0.0f;
+                    }
+#pragma unroll
+                    for (int m = 0; m < NCH; ++m) {
+                        if ((uint32_t)m < nchunk) {
+                            const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                            VR sv;
+#pragma unroll
+                            for (int v = 0; v < RW; ++v) {
+                                T sn;
+                                if constexpr (sizeof(T) == 8)
+                                    sn = xv * (gz[m * RW + v] / norm);
+                                else
+                                    sn = gz[m * RW + v] * scale;
+                                vset<T, RW>(sv, v, sn);
+                                acc[m * RW + v] += ACC(sn);
+                            }
+                            *reinterpret_cast<VR*>(fb + mm) = sv;
+                        }
+                    }
+                }
             } else {
                 wave_sync();
                 if (lane < nj) {
@@ -297,7 +395,7 @@ __global__ void __launch_bounds__(64 * WAVES)
                     gmut<ACC>(I->part_ipk)[((size_t)item.tj * x + i) * z + kq] = rs;
             }
 
-            if constexpr (MODE == PASS_UPDATE) {
+            if constexpr (pass_steps(MODE)) {
                 // 16-byte stores of every chunk that lies inside the row
                 // segment; the (at most two) chunks straddling its ends are
                 // written element by element so neighbours are not touched.
@@ -327,7 +425,7 @@ __global__ void __launch_bounds__(64 * WAVES)
         __syncthreads();
         ACC* const comb_self = reinterpret_cast<ACC*>(raw_base + w * Lds::raw_stride(z));
         if (lane < nj) {
-            if constexpr (MODE == PASS_UPDATE) {
+            if constexpr (pass_steps(MODE)) {
 #pragma unroll
                 for (int m = 0; m < NCH; ++m) {
                     if ((uint32_t)m < nchunk) {
@@ -475,6 +573,9 @@ __global__ void __launch_bounds__(256)
             for (uint32_t q = 0; q < P; ++q) B.add(part[q * z + tid]);
             gmut<T>(I->psiA)[tid] = psi_sel<PSI, T>(B.get());
         }
+        // the additive solver's iteration count: this kernel runs once per
+        // instance and iteration, before the iteration's pass
+        if (tid == 0 && I->iter != nullptr) I->iter[0] += 1u;
         __syncthreads();
     }
 }
@@ -645,7 +746,7 @@ static hipError_t launch_pass_zb(int zb, dim3 grid, hipStream_t st, const BldIns
 template <typename T, typename ACC, int MODE, int PSI>
 static hipError_t launch_pass_rw(int rw, int zb, dim3 grid, hipStream_t st, const BldInst* insts,
                                  const PassItem* items, const uint32_t* n, uint32_t zmax) {
-    if constexpr (MODE != PASS_UPDATE) {
+    if constexpr (!pass_steps(MODE)) {
         // sums-only and scoring passes read fibers element by element
         (void)rw;
         return launch_pass_zb<T, ACC, 1, MODE, PSI>(zb, grid, st, insts, items, n, zmax);
@@ -672,6 +773,8 @@ hipError_t launch_pass(int dtype, int psi, int mode, int zb, int rw, dim3 grid, 
                 return launch_pass_rw<float, float, PASS_SUMS, PSI_F32>(rw, zb, grid, st, insts, items, n_items, zmax);
             case PASS_SCORE:
                 return launch_pass_rw<float, double, PASS_SCORE, PSI_F32>(rw, zb, grid, st, insts, items, n_items, zmax);
+            case PASS_ADD:
+                return launch_pass_rw<float, float, PASS_ADD, PSI_F32>(rw, zb, grid, st, insts, items, n_items, zmax);
         }
     } else {
         switch (mode) {
@@ -683,6 +786,9 @@ hipError_t launch_pass(int dtype, int psi, int mode, int zb, int rw, dim3 grid, 
                 return launch_pass_rw<double, double, PASS_SUMS, PSI_F64_EXACT>(rw, zb, grid, st, insts, items, n_items, zmax);
             case PASS_SCORE:
                 return launch_pass_rw<double, double, PASS_SCORE, PSI_F64_EXACT>(rw, zb, grid, st, insts, items, n_items, zmax);
+            case PASS_ADD:
+                // the additive solver evaluates every psi with psi_appr
+                return launch_pass_rw<double, double, PASS_ADD, PSI_F64_APPR>(rw, zb, grid, st, insts, items, n_items, zmax);
         }
     }
     return hipErrorInvalidValue;

@@ -44,6 +44,8 @@ struct BldInst {
     double* score;        // score partials (log-marginal), may be null
     double* result;       // log-marginal output (device), may be null
     void* sijp;           // fiber sums S_ij+ (sums-only pass), may be null
+    uint32_t* iter;       // additive solver: iterations started (the psi(A)
+                          // kernel counts them; the step size depends on it)
     double eps;
     double a, b;          // Gamma prior (log-marginal only)
     float inv_z;          // 1.0f / z (fiber index from element index)

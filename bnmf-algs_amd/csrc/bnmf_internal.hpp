@@ -38,7 +38,7 @@ void clear_last_error();
     } while (0)
 
 // ----------------------------------------------------------------- kernels
-enum : int { MODE_UPDATE = 0, MODE_SUMS = 1, MODE_SCORE = 2 };
+enum : int { MODE_UPDATE = 0, MODE_SUMS = 1, MODE_SCORE = 2, MODE_ADD = 3 };
 enum : int { PSIM_F32 = 0, PSIM_F64_EXACT = 1, PSIM_F64_APPR = 2 };
 hipError_t launch_pass(int dtype, int psi, int mode, int zb, int rw, dim3 grid, hipStream_t st,
                        const BldInst* insts, const PassItem* items, const uint32_t* n_items,
@@ -152,7 +152,9 @@ hipError_t launch_pass_key(const PassKey& k, int dtype, int psi, int mode, dim3 
 // captured per-iteration graphs.
 class Batch {
    public:
-    Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& specs);
+    // algo: BNMF_ALGO_MULT (bld_mult) or BNMF_ALGO_ADD (bld_add)
+    Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& specs,
+          int algo = BNMF_ALGO_MULT);
     ~Batch();
     void solve_async(size_t iters);
     void synchronize();
@@ -183,10 +185,14 @@ class Batch {
     void enqueue_pass(int mode, bool first);
     // pass launches of the update pass (groups_[0]) and of the sums-only and
     // scoring passes (groups_[1])
-    std::vector<Bucket>& groups_of(int mode) { return groups_[mode == MODE_UPDATE ? 0 : 1]; }
+    std::vector<Bucket>& groups_of(int mode) {
+        return groups_[(mode == MODE_UPDATE || mode == MODE_ADD) ? 0 : 1];
+    }
     void capture_graphs();
 
     int dtype_, psi_;
+    int algo_ = BNMF_ALGO_MULT;
+    int step_mode_ = MODE_UPDATE;  // pass mode of an iteration's step
     int dev_ = 0;
     double eps_;
     size_t es_;
@@ -195,6 +201,7 @@ class Batch {
     DeviceBuffer arena_;
     DeviceBuffer meta_;  // descriptors, work lists, counts
     size_t off_desc_first_ = 0, off_desc_ = 0, off_prep_items_ = 0, off_prep_count_ = 0;
+    size_t off_iter_ = 0;  // additive solver: uint32 iteration counters, one per instance
     uint32_t n_prep_items_ = 0;
     std::vector<Bucket> groups_[2];
     // per stream: descriptor arrays of the instances of its update groups

@@ -73,7 +73,7 @@ int psi_mode(int dtype, int use_psi_appr) {
 template <typename T>
 void solve_one(int dtype, const T* X, size_t x, size_t y, size_t z, const double* alpha,
                const double* beta, size_t max_iter, int use_psi_appr, double eps, const T* S0,
-               uint64_t seed, T* S_out) {
+               uint64_t seed, T* S_out, int algo = BNMF_ALGO_MULT) {
     check_params(X, x, y, z, alpha, beta);
     BNMF_REQUIRE(x * y * z == 0 || S_out != nullptr, "S_out is null");
     if (x * y * z == 0) return;
@@ -86,7 +86,7 @@ void solve_one(int dtype, const T* X, size_t x, size_t y, size_t z, const double
     s.beta = beta;
     s.S0 = S0;
     s.seed = seed;
-    bnmf::Batch b(dtype, psi_mode(dtype, use_psi_appr), eps, {s});
+    bnmf::Batch b(dtype, psi_mode(dtype, use_psi_appr), eps, {s}, algo);
     b.solve_async(max_iter);
     b.synchronize();
     b.get_S(0, S_out);
@@ -126,6 +126,24 @@ bnmf_status bnmf_bld_mult_f32(const float* X, size_t x, size_t y, size_t z, cons
     return guarded([&] {
         solve_one<float>(BNMF_F32, X, x, y, z, alpha, beta, max_iter, use_psi_appr, eps, S0,
                          seed, S_out);
+    });
+}
+
+bnmf_status bnmf_bld_add_f64(const double* X, size_t x, size_t y, size_t z, const double* alpha,
+                             const double* beta, size_t max_iter, double eps, const double* S0,
+                             uint64_t seed, double* S_out) {
+    return guarded([&] {
+        solve_one<double>(BNMF_F64, X, x, y, z, alpha, beta, max_iter, 1, eps, S0, seed, S_out,
+                          BNMF_ALGO_ADD);
+    });
+}
+
+bnmf_status bnmf_bld_add_f32(const float* X, size_t x, size_t y, size_t z, const double* alpha,
+                             const double* beta, size_t max_iter, double eps, const float* S0,
+                             uint64_t seed, float* S_out) {
+    return guarded([&] {
+        solve_one<float>(BNMF_F32, X, x, y, z, alpha, beta, max_iter, 1, eps, S0, seed, S_out,
+                         BNMF_ALGO_ADD);
     });
 }
 
@@ -180,11 +198,18 @@ bnmf_status bnmf_tensor_sums(int dtype, const void* S, size_t x, size_t y, size_
 
 bnmf_status bnmf_batch_create(bnmf_batch** out, int dtype, const bnmf_bld_desc* descs, size_t n,
                               int use_psi_appr, double eps) {
+    return bnmf_batch_create_algo(out, dtype, BNMF_ALGO_MULT, descs, n, use_psi_appr, eps);
+}
+
+bnmf_status bnmf_batch_create_algo(bnmf_batch** out, int dtype, int algo, const bnmf_bld_desc* descs,
+                                   size_t n, int use_psi_appr, double eps) {
     return guarded([&] {
         BNMF_REQUIRE(out != nullptr, "out is null");
         *out = nullptr;
         check_dtype(dtype);
         BNMF_REQUIRE(n == 0 || descs != nullptr, "descs is null");
+        BNMF_REQUIRE(algo == BNMF_ALGO_MULT || algo == BNMF_ALGO_ADD,
+                     "algo must be BNMF_ALGO_MULT or BNMF_ALGO_ADD");
         std::vector<bnmf::InstSpec> specs(n);
         for (size_t t = 0; t < n; ++t) {
             const bnmf_bld_desc& d = descs[t];
@@ -205,7 +230,7 @@ bnmf_status bnmf_batch_create(bnmf_batch** out, int dtype, const bnmf_bld_desc* 
             s.seed = d.seed;
         }
         auto b = std::make_unique<bnmf_batch>();
-        b->impl = std::make_unique<bnmf::Batch>(dtype, psi_mode(dtype, use_psi_appr), eps, specs);
+        b->impl = std::make_unique<bnmf::Batch>(dtype, psi_mode(dtype, use_psi_appr), eps, specs, algo);
         *out = b.release();
     });
 }

@@ -12,7 +12,13 @@
 
 namespace bnmf {
 
-enum PassMode : int { PASS_UPDATE = 0, PASS_SUMS = 1, PASS_SCORE = 2 };
+// UPDATE: multiplicative step (bld_mult); SUMS: axis sums only; SCORE: sums
+// and lgamma(S+1) for the log-marginal; ADD: additive step (bld_add).
+enum PassMode : int { PASS_UPDATE = 0, PASS_SUMS = 1, PASS_SCORE = 2, PASS_ADD = 3 };
+
+// passes that read psi(beta_eph), psi(A) and a psi(alpha_eph) row and
+// rewrite S
+__host__ __device__ constexpr bool pass_steps(int mode) { return mode == PASS_UPDATE || mode == PASS_ADD; }
 
 // Element index within a run of whole fibers -> fiber index. Exact for
 // e < 2^16 and z <= 64: (e + 0.5)/z lies at least 0.5/z from an integer and
@@ -108,7 +114,7 @@ __device__ __forceinline__ void vset(typename VecR<T, RW>::type& v, int i, T val
 __host__ __device__ constexpr size_t round16(size_t b) { return (b + 15) / 16 * 16; }
 
 // LDS layout of a pass block, for an instance of depth z:
-//   [psib: 64*z T][psiA: z T][WAVES x grad_minus row: z T]   (update only)
+//   [psib: 64*z T][psiA: z T][WAVES x grad_minus row: z T]   (step passes)
 //   [WAVES x row buffer: 64*z + 2*VEC slots of max(T, ACC)]
 // A row buffer holds one tile row of S in its natural layout: fiber f,
 // element k at delta + f*z + k, delta = misalignment of the row start.
@@ -117,13 +123,13 @@ struct PassLds {
     static constexpr size_t VEC = 16 / sizeof(T);
     static constexpr size_t SLOT = sizeof(ACC) > sizeof(T) ? sizeof(ACC) : sizeof(T);
     __host__ __device__ static constexpr size_t psiA_off(uint32_t z) {
-        return MODE == 0 ? round16((size_t)64 * z * sizeof(T)) : 0;
+        return pass_steps(MODE) ? round16((size_t)64 * z * sizeof(T)) : 0;
     }
     __host__ __device__ static constexpr size_t gm_off(uint32_t z) {
-        return MODE == 0 ? psiA_off(z) + round16((size_t)z * sizeof(T)) : 0;
+        return pass_steps(MODE) ? psiA_off(z) + round16((size_t)z * sizeof(T)) : 0;
     }
     __host__ __device__ static constexpr size_t gm_stride(uint32_t z) {
-        return MODE == 0 ? round16((size_t)z * sizeof(T)) : 0;
+        return pass_steps(MODE) ? round16((size_t)z * sizeof(T)) : 0;
     }
     __host__ __device__ static constexpr size_t raw_stride(uint32_t z) {
         return round16(((size_t)64 * z + 2 * VEC) * SLOT);

@@ -30,6 +30,10 @@ typedef enum bnmf_status {
 
 typedef enum bnmf_dtype { BNMF_F32 = 0, BNMF_F64 = 1 } bnmf_dtype;
 
+/* Solver of a batch: bld::bld_mult (multiplicative updates) or bld::bld_add
+ * (additive gradient ascent, src/bld/bld_add.hpp:69-184). */
+typedef enum bnmf_algo { BNMF_ALGO_MULT = 0, BNMF_ALGO_ADD = 1 } bnmf_algo;
+
 /* Message of the last failure on this thread ("" if none). */
 const char* bnmf_last_error(void);
 /* ABI version of this header (major * 100 + minor). */
@@ -62,6 +66,22 @@ bnmf_status bnmf_bld_mult_f32(const float* X, size_t x, size_t y, size_t z,
                               const double* alpha, const double* beta,
                               size_t max_iter, int use_psi_appr, double eps,
                               const float* S0, uint64_t seed, float* S_out);
+
+/* ------------------------------------------------------------------------
+ * bld::bld_add (src/bld/bld_add.hpp:69-184): additive gradient ascent on
+ * Z = log S with S = X * softmax_k(Z), step 0.1 / (iteration + 1)^0.55.
+ * Same arguments as bnmf_bld_mult_*; every digamma is psi_appr, as in the
+ * reference (no use_psi_appr argument there either). The initial tensor is
+ * drawn as in bld_mult (the reference draws it inline, :76-91).
+ * ---------------------------------------------------------------------- */
+bnmf_status bnmf_bld_add_f64(const double* X, size_t x, size_t y, size_t z,
+                             const double* alpha, const double* beta, size_t max_iter,
+                             double eps, const double* S0, uint64_t seed, double* S_out);
+/* FP32 form: the step in float; exp(log S + t) is evaluated as S exp(t), and
+ * a fiber whose exp(Z) row is all zero (X_ij = 0) updates to 0. */
+bnmf_status bnmf_bld_add_f32(const float* X, size_t x, size_t y, size_t z,
+                             const double* alpha, const double* beta, size_t max_iter,
+                             double eps, const float* S0, uint64_t seed, float* S_out);
 
 /* alloc_model::log_marginal_S (src/alloc_model/alloc_model_funcs.hpp:574-586),
  * evaluated on the GPU in FP64. S: x*y*z of the given dtype. */
@@ -103,6 +123,11 @@ typedef struct bnmf_batch bnmf_batch;
 
 bnmf_status bnmf_batch_create(bnmf_batch** out, int dtype, const bnmf_bld_desc* descs,
                               size_t n, int use_psi_appr, double eps);
+/* A batch of the given solver (bnmf_algo); bnmf_batch_create is
+ * BNMF_ALGO_MULT. With BNMF_ALGO_ADD use_psi_appr is ignored. */
+bnmf_status bnmf_batch_create_algo(bnmf_batch** out, int dtype, int algo,
+                                   const bnmf_bld_desc* descs, size_t n, int use_psi_appr,
+                                   double eps);
 /* Restarts every instance from its S0 and runs max_iter iterations. */
 bnmf_status bnmf_batch_solve(bnmf_batch* b, size_t max_iter);
 /* Same, enqueued on the batch's stream without waiting. */

@@ -20,6 +20,8 @@
 //   alloc_model::log_marginal_S        src/alloc_model/alloc_model_funcs.hpp:574-586
 //     (terms: first :34-77, second :91-132, third :146-169, fourth :179-194)
 //   bld::bld_fact                      src/bld/bld_fact.hpp:36-80
+//   bld::bld_add                       src/bld/bld_add.hpp:69-184
+//   util::normalize (L1, along k)      src/util/util.hpp:321-383
 //
 // The reference's exact digamma is GSL's gsl_sf_psi (util_details.hpp:71-73),
 // which is absent from this machine. oracle_psi below restates the digamma
@@ -251,6 +253,61 @@ void iterate(const double* X, const Dims& d, const double* alpha,
     }
 }
 
+// One iteration of bld::bld_add (src/bld/bld_add.hpp:108-181), in place on
+// S, at iteration number eph (the step size depends on it). Per fiber, in the
+// reference's operation order:
+//   grad_k = psi_appr(beta_eph_jk) - psi_appr(S_k + 1)
+//            - psi_appr(sum_i alpha_eph_ik) + psi_appr(alpha_eph_ik)  (:137-146)
+//   S_mult = sum_k S_k grad_k                                         (:148-155)
+//   Z_k    = (log S_k + eps) + eta(eph) S_k (X grad_k - S_mult)       (:150,158-166)
+//   Z_k    = exp(Z_k) / (sum_k |exp(Z_k)| + 1e-50)   util::normalize's default
+//            eps (util.hpp:321-383), not the solver's eps           (:167-173)
+//   S_k    = X Z_k                                                    (:175-181)
+// eta(step) = 0.1 / (step + 1)^0.55 (:101-103). Every psi is psi_appr.
+// With grad_out non-null, grad_S of the iteration is stored there.
+void iterate_add(const double* X, const Dims& d, const double* alpha, const double* beta,
+                 double eps, size_t eph, double* S, Work& w, double* grad_out) {
+    const long x = (long)d.x, y = (long)d.y;
+    const size_t z = d.z;
+    tensor_sums(S, d, w.S_pjk.data(), w.S_ipk.data(), nullptr);
+    for (long i = 0; i < x; ++i)
+        for (size_t k = 0; k < z; ++k)
+            w.alpha_eph[(size_t)i * z + k] = alpha[i] + w.S_ipk[(size_t)i * z + k];
+    for (long j = 0; j < y; ++j)
+        for (size_t k = 0; k < z; ++k)
+            w.beta_eph[(size_t)j * z + k] = beta[k] + w.S_pjk[(size_t)j * z + k];
+    std::vector<double> alpha_eph_sum(z, 0.0);
+    for (long i = 0; i < x; ++i)
+        for (size_t k = 0; k < z; ++k) alpha_eph_sum[k] += w.alpha_eph[(size_t)i * z + k];
+    const double eta = 0.1 / std::pow((double)(eph + 1), 0.55);
+#pragma omp parallel
+    {
+        std::vector<double> g(z), e(z);
+#pragma omp for schedule(static)
+        for (long i = 0; i < x; ++i) {
+            for (long j = 0; j < y; ++j) {
+                double* s = S + d.at(i, j, 0);
+                const double xv = X[(size_t)i * d.y + j];
+                for (size_t k = 0; k < z; ++k)
+                    g[k] = psi_appr(w.beta_eph[(size_t)j * z + k]) - psi_appr(s[k] + 1) -
+                           psi_appr(alpha_eph_sum[k]) + psi_appr(w.alpha_eph[(size_t)i * z + k]);
+                double s_mult = 0.0;
+                for (size_t k = 0; k < z; ++k) s_mult += s[k] * g[k];
+                double norm = 0.0;
+                for (size_t k = 0; k < z; ++k) {
+                    double zk = std::log(s[k]) + eps;
+                    zk += eta * s[k] * (xv * g[k] - s_mult);
+                    e[k] = std::exp(zk);
+                    norm += std::fabs(e[k]);
+                }
+                norm += 1e-50;
+                if (grad_out) std::memcpy(grad_out + d.at(i, j, 0), g.data(), z * sizeof(double));
+                for (size_t k = 0; k < z; ++k) s[k] = xv * (e[k] / norm);
+            }
+        }
+    }
+}
+
 void make_work(const Dims& d, Work& w, bool keep_grad_plus) {
     w.S_pjk.assign(d.y * d.z, 0.0);
     w.S_ipk.assign(d.x * d.z, 0.0);
@@ -309,6 +366,37 @@ int oracle_bld_mult(const double* X, size_t x, size_t y, size_t z,
     std::memcpy(S, S0, x * y * z * sizeof(double));
     for (size_t eph = 0; eph < max_iter; ++eph)
         iterate(X, d, alpha, beta, use_psi_appr, eps, S, w, false);
+    return 0;
+}
+
+// bld::bld_add (src/bld/bld_add.hpp:69-184) from an explicit S0 (the
+// reference draws it inline from GSL's taus stream, :76-91, as init_S does).
+// Returns 0 on success, 1 for a parameter error.
+int oracle_bld_add(const double* X, size_t x, size_t y, size_t z, const double* alpha,
+                   const double* beta, size_t max_iter, double eps, const double* S0, double* S,
+                   int nthreads) {
+    if (!X || !alpha || !beta || !S0 || !S) return 1;
+    if (check_X(X, x * y)) return 1;
+    set_threads(nthreads);
+    const Dims d{x, y, z};
+    Work w;
+    make_work(d, w, false);
+    std::memcpy(S, S0, x * y * z * sizeof(double));
+    for (size_t eph = 0; eph < max_iter; ++eph) iterate_add(X, d, alpha, beta, eps, eph, S, w, nullptr);
+    return 0;
+}
+
+// One bld_add iteration at iteration number eph; grad_S may be null.
+int oracle_bld_add_step(const double* X, size_t x, size_t y, size_t z, const double* alpha,
+                        const double* beta, double eps, size_t eph, const double* S_in,
+                        double* S_out, double* grad_S) {
+    if (!X || !alpha || !beta || !S_in || !S_out) return 1;
+    if (check_X(X, x * y)) return 1;
+    const Dims d{x, y, z};
+    Work w;
+    make_work(d, w, false);
+    std::memcpy(S_out, S_in, x * y * z * sizeof(double));
+    iterate_add(X, d, alpha, beta, eps, eph, S_out, w, grad_S);
     return 0;
 }
 

@@ -43,6 +43,12 @@ def lib():
         L.oracle_bld_mult.restype = ctypes.c_int
         L.oracle_bld_mult.argtypes = [_D, _SZ, _SZ, _SZ, _D, _D, _SZ, ctypes.c_int,
                                       ctypes.c_double, _D, _D, ctypes.c_int]
+        L.oracle_bld_add.restype = ctypes.c_int
+        L.oracle_bld_add.argtypes = [_D, _SZ, _SZ, _SZ, _D, _D, _SZ, ctypes.c_double, _D, _D,
+                                     ctypes.c_int]
+        L.oracle_bld_add_step.restype = ctypes.c_int
+        L.oracle_bld_add_step.argtypes = [_D, _SZ, _SZ, _SZ, _D, _D, ctypes.c_double, _SZ, _D, _D,
+                                          _D]
         L.oracle_bld_mult_step.restype = ctypes.c_int
         L.oracle_bld_mult_step.argtypes = [_D, _SZ, _SZ, _SZ, _D, _D, ctypes.c_int, ctypes.c_double,
                                            _D, _D, _D, _D, _D, _D, _D, _D, _D]
@@ -117,6 +123,30 @@ def bld_mult_step(X, alpha, beta, S_in, use_psi_appr=False, eps=1e-50):
     if rc != 0:
         raise ValueError("oracle_bld_mult_step: invalid parameters")
     return out
+
+
+def bld_add(X, z, alpha, beta, S0, max_iter=1000, eps=1e-50, nthreads=0):
+    X, alpha, beta, S0 = _f64(X), _f64(alpha), _f64(beta), _f64(S0)
+    x, y = X.shape
+    assert S0.shape == (x, y, z) and alpha.shape == (x,) and beta.shape == (z,)
+    S = np.empty((x, y, z))
+    rc = lib().oracle_bld_add(_p(X), x, y, z, _p(alpha), _p(beta), int(max_iter), float(eps),
+                              _p(S0), _p(S), int(nthreads))
+    if rc != 0:
+        raise ValueError("oracle_bld_add: invalid parameters")
+    return S
+
+
+def bld_add_step(X, alpha, beta, S_in, eph=0, eps=1e-50):
+    """One bld_add iteration at iteration number eph; returns (S_out, grad_S)."""
+    X, alpha, beta, S_in = _f64(X), _f64(alpha), _f64(beta), _f64(S_in)
+    x, y, z = S_in.shape
+    S, G = np.empty((x, y, z)), np.empty((x, y, z))
+    rc = lib().oracle_bld_add_step(_p(X), x, y, z, _p(alpha), _p(beta), float(eps), int(eph),
+                                   _p(S_in), _p(S), _p(G))
+    if rc != 0:
+        raise ValueError("oracle_bld_add_step: invalid parameters")
+    return S, G
 
 
 def log_marginal_S(S, alpha, beta, a, b) -> float:
