@@ -186,7 +186,7 @@ PassKey pass_key(int dtype, int mode, const InstLayout& L, bool nonneg) {
         return e != nullptr && e[0] == '1';
     }();
     PassKey k;
-    if (dtype == BNMF_F32 && mode == MODE_UPDATE && nonneg && !generic) {
+    if (dtype == BNMF_F32 && (mode == MODE_UPDATE || mode == MODE_ADD) && nonneg && !generic) {
         k.zexact = (int)L.z;
         return k;
     }
@@ -199,8 +199,8 @@ hipError_t launch_pass_key(const PassKey& k, int dtype, int psi, int mode, dim3 
                            hipStream_t st, const BldInst* insts, const PassItem* items,
                            const uint32_t* n_items, uint32_t zmax) {
     if (k.zexact > 0) {
-        if (dtype != BNMF_F32 || mode != MODE_UPDATE) return hipErrorInvalidValue;
-        return launch_pass_f32z(k.zexact, grid, st, insts, items, n_items);
+        if (dtype != BNMF_F32 || (mode != MODE_UPDATE && mode != MODE_ADD)) return hipErrorInvalidValue;
+        return launch_pass_f32z(k.zexact, mode, grid, st, insts, items, n_items);
     }
     return launch_pass(dtype, psi, mode, k.zb, k.rw, grid, st, insts, items, n_items, zmax);
 }

@@ -261,9 +261,11 @@ __global__ void __launch_bounds__(64 * WAVES)
                 }
             } else if constexpr (MODE == PASS_ADD) {
                 // bld::bld_add's iteration (src/bld/bld_add.hpp:137-181) per
-                // fiber; the LDS row holds psi(alpha_eph(i,k)) here
+                // fiber. The LDS row holds psi(alpha_eph(i,k)) in FP64 (the
+                // reference's operation order) and psi(A) - psi(alpha_eph)
+                // in FP32 (the exact-z kernel's order)
                 const BNMF_GLOBAL T* __restrict__ psia_g = gconst<T>(I->psia) + (size_t)i * z;
-                if (lane < z) gm_l[lane] = psia_g[lane];
+                if (lane < z) gm_l[lane] = sizeof(T) == 8 ? psia_g[lane] : psiA_l[lane] - psia_g[lane];
                 T xv = T(0);
                 if (lane < nj) xv = gconst<T>(I->X)[(size_t)i * y + j0 + lane];
                 wave_sync();
@@ -285,14 +287,17 @@ __global__ void __launch_bounds__(64 * WAVES)
 #pragma unroll
                             for (int v = 0; v < RW; ++v) {
                                 const T sk = vget<T, RW>(sv, v);
-                                T ps;
-                                if constexpr (sizeof(T) == 8)
-                                    ps = psi_sp1<PSI, T>(sk);
-                                else
-                                    ps = fast_psi ? psi_ge1_f32(sk + 1.0f) : psi_sp1<PSI_F32, float>(sk);
-                                const T g = ((vget<T, RW>(pb, v) - ps) - vget<T, RW>(pA, v)) + vget<T, RW>(pa, v);
-                                gz[m * RW + v] = g;
-                                smult += sk * g;
+                                if constexpr (sizeof(T) == 8) {
+                                    const T ps = psi_sp1<PSI, T>(sk);
+                                    const T g = ((vget<T, RW>(pb, v) - ps) - vget<T, RW>(pA, v)) + vget<T, RW>(pa, v);
+                                    gz[m * RW + v] = g;
+                                    smult += sk * g;
+                                } else {
+                                    const float ps = fast_psi ? psi_ge1_f32(sk + 1.0f) : psi_sp1<PSI_F32, float>(sk);
+                                    const float g = (vget<T, RW>(pb, v) - ps) - vget<T, RW>(pa, v);
+                                    gz[m * RW + v] = g;
+                                    smult = fmaf(sk, g, smult);
+                                }
                             }
                         }
                     }
@@ -308,16 +313,16 @@ __global__ void __launch_bounds__(64 * WAVES)
 #pragma unroll
                             for (int v = 0; v < RW; ++v) {
                                 const T sk = vget<T, RW>(sv, v);
-                                const T d = xv * gz[m * RW + v] - smult;
                                 T e;
                                 if constexpr (sizeof(T) == 8) {
                                     T zk = log(sk) + eps;
-                                    zk += eta * sk * d;
+                                    zk += eta * sk * (xv * gz[m * RW + v] - smult);
                                     e = exp(zk);
                                 } else {
                                     // exp(log s + t) = s exp(t): one rounding of
                                     // s instead of log's (0 stays 0)
-                                    e = sk * __expf(static_cast<float>(eta) * sk * d);
+                                    const float etaf = static_cast<float>(eta);
+                                    e = sk * __expf((etaf * sk) * fmaf(xv, gz[m * RW + v], -smult));
                                 }
                                 gz[m * RW + v] = e;
                                 norm += e;

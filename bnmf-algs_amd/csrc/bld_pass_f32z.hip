@@ -1,5 +1,6 @@
-// bld_pass_f32z.hip — the FP32-bulk update pass specialised on the exact
-// fiber depth z (1..64).
+// bld_pass_f32z.hip — the FP32-bulk step passes (bld_mult's multiplicative
+// update and bld_add's additive one) specialised on the exact fiber depth z
+// (1..64).
 //
 // Same work decomposition, LDS layout and arithmetic as the generic update
 // pass in bld_kernels.hip (a block of WAVES wavefronts per tile of ti_rows x
@@ -102,6 +103,9 @@ constexpr int f32z_waves() {
 #ifndef BNMF_F32Z_WPE
 #define BNMF_F32Z_WPE(Z) ((Z) <= 12 ? 4 : (Z) <= 20 ? 3 : ((Z) <= 32 || (Z) == 36) ? 2 : 1)
 #endif
+// The additive step keeps more live values per element (exp(Z) and the
+// row's scale): one wave fewer where the multiplicative budget would spill.
+#define BNMF_F32Z_WPE_ADD(Z) ((Z) <= 7 ? 4 : (Z) <= 16 ? 3 : (Z) <= 32 ? 2 : 1)
 
 // Memory pipelining: a wave issues the next row's loads (S chunks, X, psia)
 // right after moving the current row into LDS, and every global store of a
@@ -109,8 +113,11 @@ constexpr int f32z_waves() {
 // out-of-range offset, which the hardware drops). The instruction count between a row's loads and their
 // first use is then fixed, so the wait for them leaves the previous row's
 // stores in flight instead of draining the whole memory queue.
-template <int Z, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BNMF_F32Z_WPE(Z))))
+// MODE: PASS_UPDATE (bld_mult's multiplicative step) or PASS_ADD (bld_add's
+// additive step, src/bld/bld_add.hpp:137-181).
+template <int Z, int WAVES, int MODE>
+__global__ void __launch_bounds__(64 * WAVES)
+    __attribute__((amdgpu_waves_per_eu(MODE == PASS_ADD ? BNMF_F32Z_WPE_ADD(Z) : BNMF_F32Z_WPE(Z))))
     bld_pass_f32z_kernel(const BldInst* __restrict__ insts, const PassItem* __restrict__ items,
                          const uint32_t* __restrict__ n_items_p) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -242,6 +249,11 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
             __syncthreads();
         }
 
+        // additive step size 0.1 / (iteration + 1)^0.55 (bld_add.hpp:101-103);
+        // *iter counts the iterations started, this one included
+        float etaf = 0.0f;
+        if constexpr (MODE == PASS_ADD) etaf = static_cast<float>(0.1 / pow((double)*I->iter, 0.55));
+
         float acc[Z];
 #pragma unroll
         for (int k = 0; k < Z; ++k) acc[k] = 0.0f;
@@ -272,103 +284,211 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                 const float* const pbb = psib_l + lane * Lds::PBS;
                 float gp[Z];
                 float sv_keep[KEEP ? Z : 1];
-                float sumS = 0.0f, nom = 0.0f, den = 0.0f;
-                // Phase A: per element g = psi(beta_eph) - psi(S+1) and the
-                // fiber sums S_ij+, sum_k S*gm and sum_k S*g.
-                // S >= 0 throughout (S0 >= 0 and beta >= 1), so S + 1 >= 1
-                // and psi takes its x >= 1 form directly.
-                auto elem_a = [&](int k, float sk, float pbk, float gmk) {
-                    if constexpr (KEEP) sv_keep[k] = sk;
-                    // psi(S+1) rounded at its own magnitude first: forming
-                    // pbk - ln2*log2(S+1) before adding the small polynomial
-                    // term would round at the (larger) magnitude of
-                    // psi(beta_eph), which ill-conditioned instances amplify
-                    const float g = pbk - psi_ge1_f32(sk + 1.0f);
-                    gp[k] = g;
-                    sumS += sk;
-                    nom = fmaf(sk, gmk, nom);
-                    den = fmaf(sk, g, den);
-                };
-                if constexpr (!PAD) {
+                if constexpr (MODE == PASS_ADD) {
+                    // bld_add (src/bld/bld_add.hpp:137-181). Phase A:
+                    // grad_S = psi(beta_eph) - psi(S+1) - (psi(A) - psi(alpha_eph))
+                    // and S_mult = sum_k S grad_S
+                    float smult = 0.0f;
+                    auto elem_a = [&](int k, float sk, float pbk, float gmk) {
+                        if constexpr (KEEP) sv_keep[k] = sk;
+                        const float g = (pbk - psi_ge1_f32(sk + 1.0f)) - gmk;
+                        gp[k] = g;
+                        smult = fmaf(sk, g, smult);
+                    };
+                    if constexpr (!PAD) {
 #pragma unroll
-                    for (int m = 0; m < NCH; ++m) {
-                        const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
-                        const VR sv = *reinterpret_cast<const VR*>(fb + mm);
-                        const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
-                        const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+                        for (int m = 0; m < NCH; ++m) {
+                            const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                            const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                            const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                            const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
 #pragma unroll
-                        for (int v = 0; v < RW; ++v)
-                            elem_a(m * RW + v, vget<float, RW>(sv, v), vget<float, RW>(pb, v),
-                                   vget<float, RW>(gv, v));
-                    }
-                } else {
-                    // z % 4 != 0: the fiber in RW-wide reads, psi(beta_eph)
-                    // and gm 16 bytes at a time
-#pragma unroll
-                    for (int g4 = 0; g4 < NG; ++g4) {
-                        const V16 pb = *reinterpret_cast<const V16*>(pbb + 4 * g4);
-                        const V16 gv = *reinterpret_cast<const V16*>(gm_l + 4 * g4);
-                        float s4[4];
-                        load4(fb, g4, s4);
-#pragma unroll
-                        for (int v = 0; v < 4; ++v)
-                            if (4 * g4 + v < Z) elem_a(4 * g4 + v, s4[v], pb[v], gv[v]);
-                    }
-                }
-                // Per-fiber constants. With Xh = 1/(X + eps) (X_reciprocal)
-                // the reference's nom and denom are Xh * sum, and its update
-                //   S *= (g + nom) * X / (S_ij+ + eps) / (gm + den + eps)
-                // is S *= (g + nom) / (gm * q + (den + eps) * q) with
-                // q = (S_ij+ + eps) / X. eps = 1e-50 flushes to zero in
-                // float; the reference's double-promoted guard makes an
-                // X_ij = 0 or all-zero fiber update to exactly 0, taken here
-                // as q = 0, (den + eps) * q = inf: the factor is then 0.
-                const bool upd = xv > 0.0f && sumS != 0.0f;
-                const float xh = upd ? 1.0f / (xv + epsf) : 0.0f;
-                nom *= xh;
-                den *= xh;
-                const float q = upd ? (sumS + epsf) / xv : 0.0f;
-                const float dq = upd ? (den + epsf) * q : __builtin_inff();
-                // Phase B: the multiplicative update and the S_+jk partials
-                auto elem_b = [&](int k, float sk, float gmk) {
-                    const float r = recip_fast<float>(fmaf(gmk, q, dq));
-                    const float sn = sk * ((gp[k] + nom) * r);
-                    acc[k] += sn;
-                    return sn;
-                };
-                if constexpr (!PAD) {
-#pragma unroll
-                    for (int m = 0; m < NCH; ++m) {
-                        const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
-                        VR sv;
-                        if constexpr (KEEP) {
-#pragma unroll
-                            for (int v = 0; v < RW; ++v) vset<float, RW>(sv, v, sv_keep[m * RW + v]);
-                        } else {
-                            sv = *reinterpret_cast<const VR*>(fb + mm);
+                            for (int v = 0; v < RW; ++v)
+                                elem_a(m * RW + v, vget<float, RW>(sv, v), vget<float, RW>(pb, v),
+                                       vget<float, RW>(gv, v));
                         }
-                        const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+                    } else {
 #pragma unroll
-                        for (int v = 0; v < RW; ++v)
-                            vset<float, RW>(sv, v, elem_b(m * RW + v, vget<float, RW>(sv, v),
-                                                          vget<float, RW>(gv, v)));
-                        *reinterpret_cast<VR*>(fb + mm) = sv;
-                    }
-                } else {
-#pragma unroll
-                    for (int g4 = 0; g4 < NG; ++g4) {
-                        const V16 gv = *reinterpret_cast<const V16*>(gm_l + 4 * g4);
-                        float s4[4];
-                        if constexpr (KEEP) {
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) s4[v] = 4 * g4 + v < Z ? sv_keep[4 * g4 + v] : 0.0f;
-                        } else {
+                        for (int g4 = 0; g4 < NG; ++g4) {
+                            const V16 pb = *reinterpret_cast<const V16*>(pbb + 4 * g4);
+                            const V16 gv = *reinterpret_cast<const V16*>(gm_l + 4 * g4);
+                            float s4[4];
                             load4(fb, g4, s4);
-                        }
 #pragma unroll
-                        for (int v = 0; v < 4; ++v)
-                            if (4 * g4 + v < Z) s4[v] = elem_b(4 * g4 + v, s4[v], gv[v]);
-                        store4(fb, g4, s4);
+                            for (int v = 0; v < 4; ++v)
+                                if (4 * g4 + v < Z) elem_a(4 * g4 + v, s4[v], pb[v], gv[v]);
+                        }
+                    }
+                    // Phase B: exp(Z) = exp(log S + eta S (X grad_S - S_mult))
+                    // = S exp(eta S (X grad_S - S_mult)) (one rounding of S
+                    // instead of log's; S = 0 stays 0) and its L1 norm
+                    float norm = 0.0f;
+                    auto elem_e = [&](int k, float sk) {
+                        const float e = sk * __expf((etaf * sk) * fmaf(xv, gp[k], -smult));
+                        gp[k] = e;
+                        norm += e;
+                    };
+                    if constexpr (!PAD) {
+#pragma unroll
+                        for (int m = 0; m < NCH; ++m) {
+                            const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                            VR sv;
+                            if constexpr (KEEP) {
+#pragma unroll
+                                for (int v = 0; v < RW; ++v) vset<float, RW>(sv, v, sv_keep[m * RW + v]);
+                            } else {
+                                sv = *reinterpret_cast<const VR*>(fb + mm);
+                            }
+#pragma unroll
+                            for (int v = 0; v < RW; ++v) elem_e(m * RW + v, vget<float, RW>(sv, v));
+                        }
+                    } else {
+#pragma unroll
+                        for (int g4 = 0; g4 < NG; ++g4) {
+                            float s4[4];
+                            if constexpr (KEEP) {
+#pragma unroll
+                                for (int v = 0; v < 4; ++v) s4[v] = 4 * g4 + v < Z ? sv_keep[4 * g4 + v] : 0.0f;
+                            } else {
+                                load4(fb, g4, s4);
+                            }
+#pragma unroll
+                            for (int v = 0; v < 4; ++v)
+                                if (4 * g4 + v < Z) elem_e(4 * g4 + v, s4[v]);
+                        }
+                    }
+                    // Phase C: S = X exp(Z) / |exp(Z)|_1 (the reference's
+                    // 1e-50 guard underflows in float: an all-zero row, X = 0,
+                    // is taken explicitly) and the S_+jk partials
+                    const float scale = norm > 0.0f ? xv / norm : 0.0f;
+                    if constexpr (!PAD) {
+#pragma unroll
+                        for (int m = 0; m < NCH; ++m) {
+                            const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                            VR sv;
+#pragma unroll
+                            for (int v = 0; v < RW; ++v) {
+                                const float sn = gp[m * RW + v] * scale;
+                                acc[m * RW + v] += sn;
+                                vset<float, RW>(sv, v, sn);
+                            }
+                            *reinterpret_cast<VR*>(fb + mm) = sv;
+                        }
+                    } else {
+#pragma unroll
+                        for (int g4 = 0; g4 < NG; ++g4) {
+                            float s4[4];
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int k = 4 * g4 + v;
+                                s4[v] = 0.0f;
+                                if (k < Z) {
+                                    s4[v] = gp[k] * scale;
+                                    acc[k] += s4[v];
+                                }
+                            }
+                            store4(fb, g4, s4);
+                        }
+                    }
+                } else {
+                    float sumS = 0.0f, nom = 0.0f, den = 0.0f;
+                    // Phase A: per element g = psi(beta_eph) - psi(S+1) and the
+                    // fiber sums S_ij+, sum_k S*gm and sum_k S*g.
+                    // S >= 0 throughout (S0 >= 0 and beta >= 1), so S + 1 >= 1
+                    // and psi takes its x >= 1 form directly.
+                    auto elem_a = [&](int k, float sk, float pbk, float gmk) {
+                        if constexpr (KEEP) sv_keep[k] = sk;
+                        // psi(S+1) rounded at its own magnitude first: forming
+                        // pbk - ln2*log2(S+1) before adding the small polynomial
+                        // term would round at the (larger) magnitude of
+                        // psi(beta_eph), which ill-conditioned instances amplify
+                        const float g = pbk - psi_ge1_f32(sk + 1.0f);
+                        gp[k] = g;
+                        sumS += sk;
+                        nom = fmaf(sk, gmk, nom);
+                        den = fmaf(sk, g, den);
+                    };
+                    if constexpr (!PAD) {
+    #pragma unroll
+                        for (int m = 0; m < NCH; ++m) {
+                            const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                            const VR sv = *reinterpret_cast<const VR*>(fb + mm);
+                            const VR pb = *reinterpret_cast<const VR*>(pbb + mm);
+                            const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+    #pragma unroll
+                            for (int v = 0; v < RW; ++v)
+                                elem_a(m * RW + v, vget<float, RW>(sv, v), vget<float, RW>(pb, v),
+                                       vget<float, RW>(gv, v));
+                        }
+                    } else {
+                        // z % 4 != 0: the fiber in RW-wide reads, psi(beta_eph)
+                        // and gm 16 bytes at a time
+    #pragma unroll
+                        for (int g4 = 0; g4 < NG; ++g4) {
+                            const V16 pb = *reinterpret_cast<const V16*>(pbb + 4 * g4);
+                            const V16 gv = *reinterpret_cast<const V16*>(gm_l + 4 * g4);
+                            float s4[4];
+                            load4(fb, g4, s4);
+    #pragma unroll
+                            for (int v = 0; v < 4; ++v)
+                                if (4 * g4 + v < Z) elem_a(4 * g4 + v, s4[v], pb[v], gv[v]);
+                        }
+                    }
+                    // Per-fiber constants. With Xh = 1/(X + eps) (X_reciprocal)
+                    // the reference's nom and denom are Xh * sum, and its update
+                    //   S *= (g + nom) * X / (S_ij+ + eps) / (gm + den + eps)
+                    // is S *= (g + nom) / (gm * q + (den + eps) * q) with
+                    // q = (S_ij+ + eps) / X. eps = 1e-50 flushes to zero in
+                    // float; the reference's double-promoted guard makes an
+                    // X_ij = 0 or all-zero fiber update to exactly 0, taken here
+                    // as q = 0, (den + eps) * q = inf: the factor is then 0.
+                    const bool upd = xv > 0.0f && sumS != 0.0f;
+                    const float xh = upd ? 1.0f / (xv + epsf) : 0.0f;
+                    nom *= xh;
+                    den *= xh;
+                    const float q = upd ? (sumS + epsf) / xv : 0.0f;
+                    const float dq = upd ? (den + epsf) * q : __builtin_inff();
+                    // Phase B: the multiplicative update and the S_+jk partials
+                    auto elem_b = [&](int k, float sk, float gmk) {
+                        const float r = recip_fast<float>(fmaf(gmk, q, dq));
+                        const float sn = sk * ((gp[k] + nom) * r);
+                        acc[k] += sn;
+                        return sn;
+                    };
+                    if constexpr (!PAD) {
+    #pragma unroll
+                        for (int m = 0; m < NCH; ++m) {
+                            const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                            VR sv;
+                            if constexpr (KEEP) {
+    #pragma unroll
+                                for (int v = 0; v < RW; ++v) vset<float, RW>(sv, v, sv_keep[m * RW + v]);
+                            } else {
+                                sv = *reinterpret_cast<const VR*>(fb + mm);
+                            }
+                            const VR gv = *reinterpret_cast<const VR*>(gm_l + mm);
+    #pragma unroll
+                            for (int v = 0; v < RW; ++v)
+                                vset<float, RW>(sv, v, elem_b(m * RW + v, vget<float, RW>(sv, v),
+                                                              vget<float, RW>(gv, v)));
+                            *reinterpret_cast<VR*>(fb + mm) = sv;
+                        }
+                    } else {
+    #pragma unroll
+                        for (int g4 = 0; g4 < NG; ++g4) {
+                            const V16 gv = *reinterpret_cast<const V16*>(gm_l + 4 * g4);
+                            float s4[4];
+                            if constexpr (KEEP) {
+    #pragma unroll
+                                for (int v = 0; v < 4; ++v) s4[v] = 4 * g4 + v < Z ? sv_keep[4 * g4 + v] : 0.0f;
+                            } else {
+                                load4(fb, g4, s4);
+                            }
+    #pragma unroll
+                            for (int v = 0; v < 4; ++v)
+                                if (4 * g4 + v < Z) s4[v] = elem_b(4 * g4 + v, s4[v], gv[v]);
+                            store4(fb, g4, s4);
+                        }
                     }
                 }
             }
@@ -458,12 +578,12 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     }
 }
 
-template <int Z>
+template <int Z, int MODE>
 static hipError_t launch_f32z_t(dim3 grid, hipStream_t st, const BldInst* insts,
                                 const PassItem* items, const uint32_t* n_items) {
     constexpr int WAVES = f32z_waves<Z>();
     const size_t lds = F32zLds<Z>::bytes(WAVES);
-    hipLaunchKernelGGL((bld_pass_f32z_kernel<Z, WAVES>), grid, dim3(64 * WAVES), lds, st, insts,
+    hipLaunchKernelGGL((bld_pass_f32z_kernel<Z, WAVES, MODE>), grid, dim3(64 * WAVES), lds, st, insts,
                        items, n_items);
     return hipGetLastError();
 }
@@ -471,17 +591,22 @@ static hipError_t launch_f32z_t(dim3 grid, hipStream_t st, const BldInst* insts,
 template <int Z>
 static hipError_t attr_f32z_t() {
     constexpr int WAVES = f32z_waves<Z>();
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(bld_pass_f32z_kernel<Z, WAVES>),
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(bld_pass_f32z_kernel<Z, WAVES, PASS_UPDATE>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)F32zLds<Z>::bytes(WAVES));
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(bld_pass_f32z_kernel<Z, WAVES, PASS_ADD>),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)F32zLds<Z>::bytes(WAVES));
 }
 
 template <int... Zs>
 struct F32zTable {
+    template <int MODE>
     static hipError_t launch(int z, dim3 grid, hipStream_t st, const BldInst* insts,
                              const PassItem* items, const uint32_t* n_items) {
         hipError_t e = hipErrorInvalidValue;
-        (void)((Zs == z ? (e = launch_f32z_t<Zs>(grid, st, insts, items, n_items), true) : false) || ...);
+        (void)((Zs == z ? (e = launch_f32z_t<Zs, MODE>(grid, st, insts, items, n_items), true) : false) || ...);
         return e;
     }
     static hipError_t attrs() {
@@ -496,10 +621,13 @@ using F32zAll = F32zTable<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16,
                           41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59,
                           60, 61, 62, 63, 64>;
 
-// FP32 update pass of instances that all have depth z (1..64).
-hipError_t launch_pass_f32z(int z, dim3 grid, hipStream_t st, const BldInst* insts,
+// FP32 step pass (mode PASS_UPDATE or PASS_ADD) of instances that all have
+// depth z (1..64).
+hipError_t launch_pass_f32z(int z, int mode, dim3 grid, hipStream_t st, const BldInst* insts,
                             const PassItem* items, const uint32_t* n_items) {
-    return F32zAll::launch(z, grid, st, insts, items, n_items);
+    if (mode == PASS_ADD) return F32zAll::launch<PASS_ADD>(z, grid, st, insts, items, n_items);
+    if (mode == PASS_UPDATE) return F32zAll::launch<PASS_UPDATE>(z, grid, st, insts, items, n_items);
+    return hipErrorInvalidValue;
 }
 
 hipError_t init_f32z_attributes() { return F32zAll::attrs(); }

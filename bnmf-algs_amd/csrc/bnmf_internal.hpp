@@ -57,8 +57,9 @@ hipError_t launch_init_s(int dtype, void* S, const void* X, size_t n_fibers, uin
 hipError_t init_kernel_attributes();
 hipError_t launch_fact(int dtype, const BldInst* inst, uint32_t y, uint32_t z, double eps, double* W,
                        double* H, double* L, hipStream_t st);
-// FP32 update pass specialised on the exact fiber depth z (bld_pass_f32z.hip)
-hipError_t launch_pass_f32z(int z, dim3 grid, hipStream_t st, const BldInst* insts,
+// FP32 step pass (MODE_UPDATE or MODE_ADD) specialised on the exact fiber
+// depth z (bld_pass_f32z.hip)
+hipError_t launch_pass_f32z(int z, int mode, dim3 grid, hipStream_t st, const BldInst* insts,
                             const PassItem* items, const uint32_t* n_items);
 hipError_t init_f32z_attributes();
 void ensure_runtime();  // per-process kernel attributes, once

@@ -208,7 +208,7 @@ static void bld_add_algorithm_checks() {
     double err = 0;
     for (long t = 0; t < S.size(); ++t) err = std::max(err, std::fabs((double)Sf.data()[t] - S.data()[t]));
     std::printf("bld_add float vs double max |dS| %.3e\n", err);
-    CHECK(err <= 1e-3 * 19);  // 1e-3 relative to max X
+    CHECK(err <= 1e-5 * 19);  // 1e-5 relative to max X
 }
 
 int main(int argc, char** argv) {

@@ -5,7 +5,8 @@ Tolerances (per test):
   * FP64 form vs the oracle: 1e-12 relative normwise after one and two steps
     (the step size depends on the iteration number, so these pin its
     indexing), 1e-10 after whole solves;
-  * FP32 form vs the FP64 oracle: 2e-4 relative normwise after 200 steps;
+  * FP32 form vs the FP64 oracle: 1e-5 relative normwise after 200 steps
+    (measured on MI355X: 5e-7 to 1e-6 on these shapes);
   * batched vs alone: bitwise; a second solve of a batch repeats the first
     bitwise (the step counter restarts).
 """
@@ -85,14 +86,17 @@ def test_add_zero_and_degenerate_inputs(hip):
     assert np.allclose(S1[:, :, 0], X, rtol=1e-14, atol=0)
 
 
+@pytest.mark.parametrize("beta", [1.0, 0.5])
 @pytest.mark.parametrize("shape", [(37, 70, 5), (70, 130, 17), (20, 65, 64), (50, 40, 31)])
-def test_add_fp32_close_to_fp64_oracle(hip, oracle, shape):
+def test_add_fp32_close_to_fp64_oracle(hip, oracle, shape, beta):
+    # beta >= 1 runs the exact-z FP32 kernel, beta < 1 the generic one
     x, y, z = shape
     p, X, S0 = _problem(hip, x, y, z, 5)
+    p = hip.Params(p.a, p.b, p.alpha, [beta] * z)
     got = hip.bld_add(X.astype(np.float32), z, p, 200, S0=S0.astype(np.float32), dtype=np.float32)
     ref = oracle.bld_add(X, z, p.alpha, p.beta, S0.astype(np.float32).astype(np.float64),
                          max_iter=200)
-    assert _rel(got, ref) < 2e-4, _rel(got, ref)
+    assert _rel(got, ref) < 1e-5, _rel(got, ref)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
