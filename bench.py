@@ -16,6 +16,8 @@ Also measured (same JSON line):
   * config1_fp64: one 1024x1024x32 instance, 500 iterations, exact digamma;
   * config2_fp32: one 2048x2048x64 instance, 300 iterations, psi_appr, with
     its normwise error and log-marginal gap against the FP64 form;
+  * bld_add_batched_fp32: the sibling additive solver (bld::bld_add) on the
+    headline's 256 instances, with the roofline of its pass;
   * roofline of the fused pass kernel of the headline batch (HIP events on
     the batch stream): algorithmic bytes 2*x*y*z*4 + x*y*4 per launch;
   * cpu_baseline: the FP64 C++/OpenMP oracle (a port of the reference CPU
@@ -236,6 +238,7 @@ def main():
                     help="serving arrival rate as a fraction of measured saturation")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-loop", action="store_true")
+    ap.add_argument("--no-add", action="store_true", help="skip the bld_add measurement")
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="plan the headline batch with the built-in tile rule")
@@ -331,6 +334,23 @@ def main():
                                    "instances": len(errs)}
         log(f"[bench] FP64 plain loop: {t64:.2f} s, {fp64['value']:.3e} upd/s; "
             f"max rel err {max(errs):.2e}, max lm gap {gaps.max():.2e}")
+
+    # ------------------------ sibling solver: bld_add on the same 256 instances
+    if not args.only_headline and not args.no_add:
+        log("[bench] batched FP32 bld_add on the same instances")
+        ba = hip.Batch(c3f, dtype=np.float32, algo=hip.ALGO_ADD)
+        ta = timed_solves(ba, iters3, 1, 1)
+        pa, ia = ba.time_pass(20)
+        ach_a = alg_bytes / (pa * 1e-3) / 1e9
+        out["bld_add_batched_fp32"] = {
+            "value": elems3 * iters3 / ta, "unit": "S-element updates/s",
+            "ms_per_solve": ta * 1e3, "pass_us": pa * 1e3, "iter_us": ia * 1e3,
+            "roofline": {"bound": "hbm", "achieved": ach_a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach_a / HBM_PEAK_GBS,
+                         "kernel": "bld_pass_f32z_kernel additive step, all launches of one iteration"},
+            "workload": "the headline's 256 instances, 200 iterations of bld::bld_add"}
+        ba.close()
+        log(f"[bench] bld_add FP32: {ta*1e3:.1f} ms/solve, pass {pa*1e3:.1f} us ({ach_a:.0f} GB/s)")
 
     # --------------------------------------------------------- config 1 (FP64)
     if not args.only_headline:
