@@ -47,6 +47,11 @@ class bnmf_bld_desc(ctypes.Structure):
                 ("seed", ctypes.c_uint64)]
 
 
+class bnmf_appr_desc(ctypes.Structure):
+    _fields_ = [("x", _SZ), ("y", _SZ), ("z", _SZ), ("X", _VP), ("alpha", _D), ("beta", _D),
+                ("nu0", _VP), ("mu0", _VP), ("seed", ctypes.c_uint64)]
+
+
 class bnmf_batch_info(ctypes.Structure):
     _fields_ = [("n_instances", _SZ), ("elements", _SZ), ("fibers", _SZ),
                 ("device_bytes", _SZ), ("pass_items", _SZ), ("prep_items", _SZ),
@@ -103,6 +108,22 @@ def lib():
                                              ctypes.POINTER(bnmf_bld_desc), _SZ, ctypes.c_int,
                                              ctypes.c_double]
         L.bnmf_batch_create_algo.restype = st
+        L.bnmf_bld_appr_f64.argtypes = [_VP, _SZ, _SZ, _SZ, _D, _D, _SZ, ctypes.c_double, _VP, _VP,
+                                        ctypes.c_uint64, _VP, _VP, _VP]
+        L.bnmf_bld_appr_f64.restype = st
+        L.bnmf_bld_appr_f32.argtypes = L.bnmf_bld_appr_f64.argtypes
+        L.bnmf_bld_appr_f32.restype = st
+        L.bnmf_appr_create.argtypes = [ctypes.POINTER(_VP), ctypes.c_int,
+                                       ctypes.POINTER(bnmf_appr_desc), _SZ, ctypes.c_double]
+        L.bnmf_appr_create.restype = st
+        L.bnmf_appr_solve.argtypes = [_VP, _SZ]
+        L.bnmf_appr_solve.restype = st
+        L.bnmf_appr_get.argtypes = [_VP, _SZ, _VP, _VP, _VP]
+        L.bnmf_appr_get.restype = st
+        L.bnmf_appr_time_iteration.argtypes = [_VP, ctypes.c_int, _D]
+        L.bnmf_appr_time_iteration.restype = st
+        L.bnmf_appr_destroy.argtypes = [_VP]
+        L.bnmf_appr_destroy.restype = None
         for name in ("bnmf_batch_solve", "bnmf_batch_solve_async"):
             getattr(L, name).argtypes = [_VP, _SZ]
             getattr(L, name).restype = st
@@ -286,6 +307,97 @@ def bld_add(X, z: int, model_params: Params, max_iter: int = 1000, eps: float = 
     _check(fn(_ptr(X), x, y, z, _dptr(alpha), _dptr(beta), int(max_iter), float(eps), _ptr(S0a),
               int(seed) & (2**64 - 1), _ptr(S)), "bld_add")
     return S
+
+
+def bld_appr(X, z: int, model_params: Params, max_iter: int = 1000, eps: float = 1e-50, *,
+             nu0=None, mu0=None, seed: int = 0, dtype=np.float64):
+    """bld::bld_appr (src/bld/bld_appr.hpp:287-372) on the GPU: returns
+    (S (x, y, z), nu (x, z), mu (z, y)). nu0 / mu0 override the seeded
+    Dirichlet(1) draws of nu's rows and mu's columns."""
+    dt = _np_dtype(dtype)
+    X = np.ascontiguousarray(X, dtype=dt)
+    _check_bld_params(X, z, model_params)
+    x, y = X.shape
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    nu0a = None if nu0 is None else np.ascontiguousarray(nu0, dtype=dt)
+    mu0a = None if mu0 is None else np.ascontiguousarray(mu0, dtype=dt)
+    if nu0a is not None and nu0a.shape != (x, z):
+        raise ValueError("nu0 must have shape (x, z)")
+    if mu0a is not None and mu0a.shape != (z, y):
+        raise ValueError("mu0 must have shape (z, y)")
+    S, nu, mu = np.empty((x, y, z), dtype=dt), np.empty((x, z), dtype=dt), np.empty((z, y), dtype=dt)
+    fn = lib().bnmf_bld_appr_f32 if dt == np.float32 else lib().bnmf_bld_appr_f64
+    _check(fn(_ptr(X), x, y, z, _dptr(alpha), _dptr(beta), int(max_iter), float(eps), _ptr(nu0a),
+              _ptr(mu0a), int(seed) & (2**64 - 1), _ptr(S), _ptr(nu), _ptr(mu)), "bld_appr")
+    return S, nu, mu
+
+
+class ApprBatch:
+    """Many bld_appr instances solved together on the GPU; the factors stay
+    resident. Instances are (X, z, params, nu0, mu0, seed) tuples or objects
+    with those attributes (nu0 / mu0 may be None)."""
+
+    def __init__(self, instances, dtype=np.float64, eps: float = 1e-50):
+        self.dt = _np_dtype(dtype)
+        arr = (bnmf_appr_desc * max(1, len(instances)))()
+        keep = []
+        self.shapes = []
+        for n, inst in enumerate(instances):
+            X = np.ascontiguousarray(inst.X, dtype=self.dt)
+            _check_bld_params(X, inst.z, inst.params)
+            alpha = np.ascontiguousarray(inst.params.alpha, dtype=np.float64)
+            beta = np.ascontiguousarray(inst.params.beta, dtype=np.float64)
+            nu0 = getattr(inst, "nu0", None)
+            mu0 = getattr(inst, "mu0", None)
+            nu0 = None if nu0 is None else np.ascontiguousarray(nu0, dtype=self.dt)
+            mu0 = None if mu0 is None else np.ascontiguousarray(mu0, dtype=self.dt)
+            keep += [X, alpha, beta, nu0, mu0]
+            d = arr[n]
+            d.x, d.y, d.z = X.shape[0], X.shape[1], inst.z
+            d.X = X.ctypes.data
+            d.alpha, d.beta = _dptr(alpha), _dptr(beta)
+            d.nu0 = None if nu0 is None else nu0.ctypes.data
+            d.mu0 = None if mu0 is None else mu0.ctypes.data
+            d.seed = int(getattr(inst, "seed", 0)) & (2**64 - 1)
+            self.shapes.append((X.shape[0], X.shape[1], inst.z))
+        h = _VP()
+        _check(lib().bnmf_appr_create(ctypes.byref(h), _code(self.dt), arr, len(instances),
+                                      float(eps)), "appr_create")
+        del keep
+        self._h = h
+
+    def solve(self, max_iter: int):
+        _check(lib().bnmf_appr_solve(self._h, int(max_iter)), "appr_solve")
+
+    def get(self, idx: int):
+        x, y, z = self.shapes[idx]
+        S = np.empty((x, y, z), dtype=self.dt)
+        nu, mu = np.empty((x, z), dtype=self.dt), np.empty((z, y), dtype=self.dt)
+        _check(lib().bnmf_appr_get(self._h, int(idx), _ptr(S), _ptr(nu), _ptr(mu)), "appr_get")
+        return S, nu, mu
+
+    def time_iteration(self, reps: int = 10) -> float:
+        out = np.zeros(1)
+        _check(lib().bnmf_appr_time_iteration(self._h, int(reps), _dptr(out)), "appr_time")
+        return float(out[0])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bnmf_appr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def log_marginal_S(S, model_params: Params) -> float:

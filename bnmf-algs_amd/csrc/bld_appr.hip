@@ -1,0 +1,436 @@
+// bld_appr.hip — kernels of the approximate BLD solver bld::bld_appr
+// (src/bld/bld_appr.hpp). S is kept in factored form, S_ijk = X_ij nu_ik
+// mu_kj / (nu mu + eps)_ij, and never stored during a solve: each pass reads
+// X (x*y) and the small factors and forms S per fiber in registers.
+//
+// One iteration (bld_appr.hpp:320-348) is two half-steps, one per factor:
+//   sums pass    X_hat = nu mu, S per fiber, per-tile partial S_+jk / S_i+k
+//                (update_X_hat, update_S and the sums of update_alpha_eph /
+//                update_beta_eph, :12-71)
+//   prep, psi(A) psi(beta_eph), psi(alpha_eph), psi(sum_i alpha_eph) (the
+//                multiplicative solver's kernels, shared)
+//   step pass    grad_plus (:73-88) and grad_minus (:90-103) per fiber and
+//                the numerator and denominator of update_nu (:105-164,
+//                reduced over j: row partials) or update_mu (:166-226,
+//                reduced over i: column partials)
+//   factor       nu *= nom / (den + eps), rows renormalised; or mu, columns
+// Tiles are ti_rows x 64 fibers as in the update pass; lane f owns column
+// j0 + f. The double sums over c of update_nu / update_mu factor per fiber:
+// sum_c nu_ic mu_cj gm_ic = A_ij and sum_c nu_ic mu_cj gp_ijc = B_ij, so
+//   nom_ik = sum_j oa_ij mu_kj gp_ijk + oa2_ij mu_kj A_ij
+//   den_ik = sum_j oa_ij mu_kj gm_ik  + oa2_ij mu_kj B_ij
+// with oa = X / (X_hat + eps), oa2 = X / (X_hat^2 + eps); update_mu has
+// nu_ik in place of mu_kj and sums over i. Every psi is psi_appr (FP64) or
+// the FP32 digamma. All reductions have a fixed order (no atomics).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bld_types.hpp"
+#include "digamma.hpp"
+#include "pass_common.hpp"
+
+namespace bnmf {
+
+enum ApprMode : int { APPR_SUMS = 0, APPR_NU = 1, APPR_MU = 2 };
+
+// k-major LDS arrays of a tile's 64 columns, padded so that the lanes of a
+// row reduction (different k, same column) fall on different banks
+constexpr uint32_t kApprStride = 65;
+
+// LDS of a block: [mu: z x 65 T][psib: z x 65 T][psiA: z T]
+// [WAVES x (nu row, gm row): 2z T][WAVES x reduction buffer: z x 65 ACC]
+template <typename T, typename ACC>
+struct ApprLds {
+    __host__ __device__ static constexpr size_t psib_off(uint32_t z) { return round16((size_t)z * kApprStride * sizeof(T)); }
+    __host__ __device__ static constexpr size_t psiA_off(uint32_t z) {
+        return psib_off(z) + round16((size_t)z * kApprStride * sizeof(T));
+    }
+    __host__ __device__ static constexpr size_t row_off(uint32_t z) { return psiA_off(z) + round16((size_t)z * sizeof(T)); }
+    __host__ __device__ static constexpr size_t row_stride(uint32_t z) { return round16((size_t)2 * z * sizeof(T)); }
+    __host__ __device__ static constexpr size_t buf_off(uint32_t z, int waves) { return row_off(z) + waves * row_stride(z); }
+    __host__ __device__ static constexpr size_t buf_stride(uint32_t z) { return round16((size_t)z * kApprStride * sizeof(ACC)); }
+    __host__ __device__ static constexpr size_t bytes(uint32_t z, int waves) { return buf_off(z, waves) + waves * buf_stride(z); }
+};
+
+template <typename T, typename ACC, int ZB>
+__host__ __device__ constexpr int appr_waves() {
+    int w = 4;
+    while (w > 1 && ApprLds<T, ACC>::bytes(ZB, w) > kLdsBytes) --w;
+    return w;
+}
+
+// Row reduction over the tile's columns: buf holds one value per (k, lane)
+// at k * 65 + lane; the result for each k < z goes to out[k]. Lanes
+// (kq, part) add strided columns in eight independent partial sums, combined
+// in a fixed tree and then across parts with xor shuffles.
+template <typename ACC>
+__device__ __forceinline__ void appr_row_reduce(const ACC* buf, uint32_t z, uint32_t zq, uint32_t zq_log2,
+                                                uint32_t nj, BNMF_GLOBAL ACC* out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t kq = lane & (zq - 1u), part = lane >> zq_log2, L = 64u >> zq_log2;
+    ACC r[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) r[v] = ACC(0);
+    if (kq < z) {
+        const ACC* const rb = buf + kq * kApprStride;
+        for (uint32_t f0 = part; f0 < nj; f0 += 8u * L) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const uint32_t f = f0 + (uint32_t)v * L;
+                if (f < nj) r[v] += rb[f];
+            }
+        }
+    }
+    ACC rs = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (uint32_t off = zq; off < 64u; off <<= 1) rs += __shfl_xor(rs, (int)off);
+    if (part == 0 && kq < z) out[kq] = rs;
+}
+
+template <typename T, typename ACC, int ZB, int MODE, int PSI, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+    bld_appr_pass_kernel(const ApprInst* __restrict__ insts, const PassItem* __restrict__ items,
+                         const uint32_t* __restrict__ n_items_p) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    using Lds = ApprLds<T, ACC>;
+    constexpr uint32_t NT = 64 * WAVES;
+    constexpr uint32_t SK = kApprStride;
+    const uint32_t n_items = *n_items_p;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
+
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const PassItem item = items[it];
+        const ApprInst* __restrict__ I = insts + item.inst;
+        const uint32_t x = I->x, y = I->y, z = I->z;
+        const uint32_t i0 = item.ti * I->ti_rows;
+        const uint32_t i1 = min(x, i0 + I->ti_rows);
+        const uint32_t j0 = item.tj * kTileCols;
+        const uint32_t nj = min(kTileCols, y - j0);
+        const T epsT = static_cast<T>(I->eps);
+
+        T* const mu_l = reinterpret_cast<T*>(lds_raw);
+        T* const psib_l = reinterpret_cast<T*>(lds_raw + Lds::psib_off(z));
+        T* const psiA_l = reinterpret_cast<T*>(lds_raw + Lds::psiA_off(z));
+        T* const nu_w = reinterpret_cast<T*>(lds_raw + Lds::row_off(z) + w * Lds::row_stride(z));
+        T* const gm_w = nu_w + z;
+        unsigned char* const buf_base = lds_raw + Lds::buf_off(z, WAVES);
+        ACC* const buf = reinterpret_cast<ACC*>(buf_base + w * Lds::buf_stride(z));
+
+        // the tile's mu columns (and psi(beta_eph) rows), k-major; columns
+        // past the instance's edge are zero
+        {
+            const BNMF_GLOBAL T* __restrict__ mu_g = gconst<T>(I->mu);
+            for (uint32_t e = tid; e < z * 64u; e += NT) {
+                const uint32_t k = e >> 6, f = e & 63u;
+                mu_l[k * SK + f] = f < nj ? mu_g[(size_t)k * y + j0 + f] : T(0);
+            }
+            if constexpr (MODE != APPR_SUMS) {
+                const BNMF_GLOBAL T* __restrict__ pb_g = gconst<T>(I->psib) + (size_t)j0 * z;
+                for (uint32_t e = tid; e < z * 64u; e += NT) {
+                    const uint32_t f = e / z, k = e - f * z;
+                    psib_l[k * SK + f] = f < nj ? pb_g[e] : T(0);
+                }
+                if (tid < z) psiA_l[tid] = gconst<T>(I->psiA)[tid];
+            }
+            __syncthreads();
+        }
+
+        // column accumulators over the wave's rows: S_+jk (sums pass), or
+        // mu's numerator and denominator (mu step)
+        ACC c1[ZB], c2[ZB];
+#pragma unroll
+        for (int k = 0; k < ZB; ++k) c1[k] = c2[k] = ACC(0);
+
+        for (uint32_t i = i0 + w; i < i1; i += WAVES) {
+            if (lane < z) {
+                nu_w[lane] = gconst<T>(I->nu)[(size_t)i * z + lane];
+                if constexpr (MODE != APPR_SUMS)
+                    gm_w[lane] = psiA_l[lane] - gconst<T>(I->psia)[(size_t)i * z + lane];
+            }
+            const T xv = lane < nj ? gconst<T>(I->X)[(size_t)i * y + j0 + lane] : T(0);
+            wave_sync();
+            // X_hat_ij = sum_k nu_ik mu_kj (update_X_hat :12-15)
+            T h = T(0);
+#pragma unroll
+            for (int k = 0; k < ZB; ++k)
+                if (k < (int)z) h += nu_w[k] * mu_l[k * SK + lane];
+            // orig_over_appr (:17-23) and orig_over_appr_squared (:25-32). In
+            // FP32 eps flushes to zero: an X_ij = 0 fiber contributes 0, as
+            // the double guard makes it in the reference
+            T oa, oa2;
+            if constexpr (sizeof(T) == 8) {
+                oa = xv / (h + epsT);
+                oa2 = xv / (h * h + epsT);
+            } else {
+                const bool live = xv > 0.0f;
+                oa = live ? xv / (h + epsT) : 0.0f;
+                oa2 = live ? xv / (h * h + epsT) : 0.0f;
+            }
+            if constexpr (MODE == APPR_SUMS) {
+                // S_ijk = oa nu_ik mu_kj (update_S :34-48) and its sums
+#pragma unroll
+                for (int k = 0; k < ZB; ++k) {
+                    if (k < (int)z) {
+                        const T s = (oa * nu_w[k]) * mu_l[k * SK + lane];
+                        c1[k] += ACC(s);
+                        buf[k * SK + lane] = ACC(s);
+                    }
+                }
+                wave_sync();
+                appr_row_reduce<ACC>(buf, z, I->zq, I->zq_log2, nj,
+                                     gmut<ACC>(I->part_ipk) + ((size_t)item.tj * x + i) * z);
+            } else {
+                // grad_plus = psi(beta_eph) - psi(S+1); A = sum_c nu mu gm,
+                // B = sum_c nu mu gp
+                T gp[ZB];
+                T A = T(0), B = T(0);
+#pragma unroll
+                for (int k = 0; k < ZB; ++k) {
+                    if (k < (int)z) {
+                        const T nk = nu_w[k], mk = mu_l[k * SK + lane];
+                        const T s = (oa * nk) * mk;
+                        T ps;
+                        if constexpr (sizeof(T) == 8)
+                            ps = psi_sel<PSI, T>(s + T(1));
+                        else
+                            ps = psi_ge1_f32(s + 1.0f);  // S >= 0
+                        const T g = psib_l[k * SK + lane] - ps;
+                        gp[k] = g;
+                        const T nm = nk * mk;
+                        A += nm * gm_w[k];
+                        B += nm * g;
+                    }
+                }
+                if constexpr (MODE == APPR_NU) {
+                    // numerator, then denominator, summed over the row's columns
+#pragma unroll
+                    for (int k = 0; k < ZB; ++k) {
+                        if (k < (int)z) {
+                            const T mk = mu_l[k * SK + lane];
+                            buf[k * SK + lane] = ACC((oa * mk) * gp[k] + (oa2 * mk) * A);
+                        }
+                    }
+                    wave_sync();
+                    appr_row_reduce<ACC>(buf, z, I->zq, I->zq_log2, nj,
+                                         gmut<ACC>(I->part_ipk) + ((size_t)item.tj * x + i) * z);
+                    wave_sync();
+#pragma unroll
+                    for (int k = 0; k < ZB; ++k) {
+                        if (k < (int)z) {
+                            const T mk = mu_l[k * SK + lane];
+                            buf[k * SK + lane] = ACC((oa * mk) * gm_w[k] + (oa2 * mk) * B);
+                        }
+                    }
+                    wave_sync();
+                    appr_row_reduce<ACC>(buf, z, I->zq, I->zq_log2, nj,
+                                         gmut<ACC>(I->part_ipk2) + ((size_t)item.tj * x + i) * z);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < ZB; ++k) {
+                        if (k < (int)z) {
+                            const T nk = nu_w[k];
+                            c1[k] += ACC((oa * nk) * gp[k] + (oa2 * nk) * A);
+                            c2[k] += ACC((oa * nk) * gm_w[k] + (oa2 * nk) * B);
+                        }
+                    }
+                }
+            }
+            wave_sync();
+        }
+
+        // column partials over the tile's rows: the waves' sums meet in LDS
+        // and are added in wave order
+        if constexpr (MODE != APPR_NU) {
+#pragma unroll
+            for (int round = 0; round < (MODE == APPR_MU ? 2 : 1); ++round) {
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < ZB; ++k)
+                    if (k < (int)z) buf[k * SK + lane] = round == 0 ? c1[k] : c2[k];
+                __syncthreads();
+                BNMF_GLOBAL ACC* __restrict__ pp =
+                    gmut<ACC>(round == 0 ? I->part_pjk : I->part_pjk2) + ((size_t)item.ti * y + j0) * z;
+                for (uint32_t e = tid; e < nj * z; e += NT) {
+                    const uint32_t f = e / z, k = e - f * z;
+                    ACC s = ACC(0);
+#pragma unroll
+                    for (int w2 = 0; w2 < WAVES; ++w2)
+                        s += reinterpret_cast<const ACC*>(buf_base + w2 * Lds::buf_stride(z))[k * SK + f];
+                    pp[e] = s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// update_nu's step and row normalisation (:155-163) or update_mu's step and
+// column normalisation (:217-225): one thread per row i (HALF 0) or column
+// j (HALF 1); the per-tile partials are added in tile order.
+template <typename T, int HALF>
+__global__ void __launch_bounds__(256)
+    bld_appr_factor_kernel(const ApprInst* __restrict__ insts, const PassItem* __restrict__ items,
+                           const uint32_t* __restrict__ n_items_p) {
+    const uint32_t n_items = *n_items_p;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const PassItem item = items[it];
+        const ApprInst* __restrict__ I = insts + item.inst;
+        const uint32_t x = I->x, y = I->y, z = I->z;
+        const uint32_t r = item.ti * 256u + threadIdx.x;
+        const T epsT = static_cast<T>(I->eps);
+        const uint32_t n_rows = HALF == 0 ? x : y;
+        if (r >= n_rows) continue;
+        // HALF 0: nu[r*z + k], partials part_ipk[(t*x + r)*z + k] over n_tj tiles
+        // HALF 1: mu[k*y + r], partials part_pjk[(t*y + r)*z + k] over n_ti tiles
+        BNMF_GLOBAL T* __restrict__ F = gmut<T>(HALF == 0 ? I->nu : I->mu);
+        const BNMF_GLOBAL T* __restrict__ Pn = gconst<T>(HALF == 0 ? I->part_ipk : I->part_pjk);
+        const BNMF_GLOBAL T* __restrict__ Pd = gconst<T>(HALF == 0 ? I->part_ipk2 : I->part_pjk2);
+        const uint32_t nt = HALF == 0 ? I->n_tj : I->n_ti;
+        const size_t tstride = (size_t)n_rows * z;
+        auto fidx = [&](uint32_t k) { return HALF == 0 ? (size_t)r * z + k : (size_t)k * y + r; };
+        SumAcc<T> tot;
+        for (uint32_t k = 0; k < z; ++k) {
+            SumAcc<T> n, d;
+            for (uint32_t t = 0; t < nt; ++t) {
+                n.add(Pn[t * tstride + (size_t)r * z + k]);
+                d.add(Pd[t * tstride + (size_t)r * z + k]);
+            }
+            T v;
+            if constexpr (sizeof(T) == 8) {
+                v = F[fidx(k)] * (n.get() / (d.get() + epsT));
+            } else {
+                // eps flushes to zero in float: a row with no data
+                // (numerator 0) keeps a zero factor, as in double
+                const float dn = d.get() + epsT;
+                v = n.get() != 0.0f ? F[fidx(k)] * (n.get() / dn) : 0.0f;
+            }
+            F[fidx(k)] = v;
+            tot.add(v);
+        }
+        T rs = tot.get() + epsT;
+        if constexpr (sizeof(T) == 4) rs = rs != 0.0f ? rs : 1.0f;
+        for (uint32_t k = 0; k < z; ++k) F[fidx(k)] = F[fidx(k)] / rs;
+    }
+}
+
+// The returned S (bld_appr.hpp:366-369): one thread per fiber.
+template <typename T>
+__global__ void __launch_bounds__(256) bld_appr_S_kernel(const ApprInst* __restrict__ I) {
+    const size_t f = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t y = I->y, z = I->z;
+    if (f >= (size_t)I->x * y) return;
+    const size_t i = f / y, j = f - i * y;
+    const BNMF_GLOBAL T* __restrict__ nu = gconst<T>(I->nu) + i * z;
+    const BNMF_GLOBAL T* __restrict__ mu = gconst<T>(I->mu) + j;
+    const T xv = gconst<T>(I->X)[f];
+    const T epsT = static_cast<T>(I->eps);
+    T h = T(0);
+    for (uint32_t k = 0; k < z; ++k) h += nu[k] * mu[(size_t)k * y];
+    T oa;
+    if constexpr (sizeof(T) == 8)
+        oa = xv / (h + epsT);
+    else
+        oa = xv > 0.0f ? xv / (h + epsT) : 0.0f;
+    BNMF_GLOBAL T* __restrict__ S = gmut<T>(I->S) + f * z;
+    for (uint32_t k = 0; k < z; ++k) S[k] = (oa * nu[k]) * mu[(size_t)k * y];
+}
+
+// ----------------------------------------------------------- launchers
+template <typename T, typename ACC, int ZB, int MODE, int PSI>
+static hipError_t launch_appr_t(dim3 grid, hipStream_t st, const ApprInst* insts, const PassItem* items,
+                                const uint32_t* n_items, uint32_t zmax) {
+    constexpr int WAVES = appr_waves<T, ACC, ZB>();
+    const size_t lds = ApprLds<T, ACC>::bytes(zmax, WAVES);
+    hipLaunchKernelGGL((bld_appr_pass_kernel<T, ACC, ZB, MODE, PSI, WAVES>), grid, dim3(64 * WAVES), lds, st,
+                       insts, items, n_items);
+    return hipGetLastError();
+}
+
+template <typename T, typename ACC, int MODE, int PSI>
+static hipError_t launch_appr_zb(int zb, dim3 grid, hipStream_t st, const ApprInst* insts,
+                                 const PassItem* items, const uint32_t* n, uint32_t zmax) {
+    switch (zb) {
+        case 8: return launch_appr_t<T, ACC, 8, MODE, PSI>(grid, st, insts, items, n, zmax);
+        case 16: return launch_appr_t<T, ACC, 16, MODE, PSI>(grid, st, insts, items, n, zmax);
+        case 32: return launch_appr_t<T, ACC, 32, MODE, PSI>(grid, st, insts, items, n, zmax);
+        case 64: return launch_appr_t<T, ACC, 64, MODE, PSI>(grid, st, insts, items, n, zmax);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// mode: 0 sums, 1 nu step, 2 mu step; dtype 0 float, 1 double
+hipError_t launch_appr_pass(int dtype, int mode, int zb, dim3 grid, hipStream_t st, const ApprInst* insts,
+                            const PassItem* items, const uint32_t* n_items, uint32_t zmax) {
+    if (dtype == 0) {
+        switch (mode) {
+            case APPR_SUMS: return launch_appr_zb<float, float, APPR_SUMS, PSI_F32>(zb, grid, st, insts, items, n_items, zmax);
+            case APPR_NU: return launch_appr_zb<float, float, APPR_NU, PSI_F32>(zb, grid, st, insts, items, n_items, zmax);
+            case APPR_MU: return launch_appr_zb<float, float, APPR_MU, PSI_F32>(zb, grid, st, insts, items, n_items, zmax);
+        }
+    } else {
+        switch (mode) {
+            case APPR_SUMS: return launch_appr_zb<double, double, APPR_SUMS, PSI_F64_APPR>(zb, grid, st, insts, items, n_items, zmax);
+            case APPR_NU: return launch_appr_zb<double, double, APPR_NU, PSI_F64_APPR>(zb, grid, st, insts, items, n_items, zmax);
+            case APPR_MU: return launch_appr_zb<double, double, APPR_MU, PSI_F64_APPR>(zb, grid, st, insts, items, n_items, zmax);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_appr_factor(int dtype, int half, dim3 grid, hipStream_t st, const ApprInst* insts,
+                              const PassItem* items, const uint32_t* n_items) {
+    if (dtype == 0) {
+        if (half == 0)
+            hipLaunchKernelGGL((bld_appr_factor_kernel<float, 0>), grid, dim3(256), 0, st, insts, items, n_items);
+        else
+            hipLaunchKernelGGL((bld_appr_factor_kernel<float, 1>), grid, dim3(256), 0, st, insts, items, n_items);
+    } else {
+        if (half == 0)
+            hipLaunchKernelGGL((bld_appr_factor_kernel<double, 0>), grid, dim3(256), 0, st, insts, items, n_items);
+        else
+            hipLaunchKernelGGL((bld_appr_factor_kernel<double, 1>), grid, dim3(256), 0, st, insts, items, n_items);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_appr_S(int dtype, const ApprInst* inst, size_t n_fibers, hipStream_t st) {
+    const dim3 grid((unsigned)((n_fibers + 255) / 256));
+    if (dtype == 0)
+        hipLaunchKernelGGL(bld_appr_S_kernel<float>, grid, dim3(256), 0, st, inst);
+    else
+        hipLaunchKernelGGL(bld_appr_S_kernel<double>, grid, dim3(256), 0, st, inst);
+    return hipGetLastError();
+}
+
+template <typename T, typename ACC, int MODE, int PSI>
+static hipError_t attr_appr_mode() {
+    hipError_t e = hipSuccess;
+    auto one = [&](auto zbc) {
+        constexpr int ZB = decltype(zbc)::value;
+        constexpr int WAVES = appr_waves<T, ACC, ZB>();
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(bld_appr_pass_kernel<T, ACC, ZB, MODE, PSI, WAVES>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)ApprLds<T, ACC>::bytes(ZB, WAVES));
+    };
+    one(std::integral_constant<int, 8>{});
+    one(std::integral_constant<int, 16>{});
+    one(std::integral_constant<int, 32>{});
+    one(std::integral_constant<int, 64>{});
+    return e;
+}
+
+hipError_t init_appr_attributes() {
+    hipError_t e = attr_appr_mode<float, float, APPR_SUMS, PSI_F32>();
+    if (e == hipSuccess) e = attr_appr_mode<float, float, APPR_NU, PSI_F32>();
+    if (e == hipSuccess) e = attr_appr_mode<float, float, APPR_MU, PSI_F32>();
+    if (e == hipSuccess) e = attr_appr_mode<double, double, APPR_SUMS, PSI_F64_APPR>();
+    if (e == hipSuccess) e = attr_appr_mode<double, double, APPR_NU, PSI_F64_APPR>();
+    if (e == hipSuccess) e = attr_appr_mode<double, double, APPR_MU, PSI_F64_APPR>();
+    return e;
+}
+
+}  // namespace bnmf

@@ -35,6 +35,7 @@ void ensure_runtime() {
     std::call_once(once, [] {
         status = init_kernel_attributes();
         if (status == hipSuccess) status = init_f32z_attributes();
+        if (status == hipSuccess) status = init_appr_attributes();
     });
     BNMF_HIP(status);
 }

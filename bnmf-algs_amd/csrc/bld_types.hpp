@@ -66,6 +66,28 @@ struct PassItem {
     uint32_t inst, ti, tj, pad;
 };
 
+// One instance of the approximate solver (bld::bld_appr): S is never stored;
+// S_ijk = X_ij nu_ik mu_kj / (nu mu + eps)_ij is formed per fiber from the
+// factors. Prep and psi(A) run on a BldInst that shares psib, psia, partA,
+// psiA and the partial buffers.
+struct ApprInst {
+    const void* X;        // x*y
+    void* nu;             // x*z, rows sum to 1
+    void* mu;             // z*y, columns sum to 1
+    void* psib;           // psi(beta_eph), y*z
+    void* psia;           // psi(alpha_eph), x*z
+    void* psiA;           // psi(sum_i alpha_eph), z
+    void* part_pjk;       // n_ti*y*z: S_+jk partials, then mu's numerator partials
+    void* part_ipk;       // n_tj*x*z: S_i+k partials, then nu's numerator partials
+    void* part_pjk2;      // n_ti*y*z: mu's denominator partials
+    void* part_ipk2;      // n_tj*x*z: nu's denominator partials
+    void* S;              // x*y*z output (final materialisation only)
+    double eps;
+    uint32_t x, y, z;
+    uint32_t ti_rows, n_ti, n_tj;
+    uint32_t zq, zq_log2;
+};
+
 // kind 0: column chunk (psib / column score terms); kind 1: row chunk
 // (psia, partA / row score terms).
 struct PrepItem {

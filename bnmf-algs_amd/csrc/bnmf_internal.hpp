@@ -62,6 +62,13 @@ hipError_t launch_fact(int dtype, const BldInst* inst, uint32_t y, uint32_t z, d
 hipError_t launch_pass_f32z(int z, int mode, dim3 grid, hipStream_t st, const BldInst* insts,
                             const PassItem* items, const uint32_t* n_items);
 hipError_t init_f32z_attributes();
+// approximate solver (bld_appr.hip): mode 0 sums, 1 nu step, 2 mu step
+hipError_t launch_appr_pass(int dtype, int mode, int zb, dim3 grid, hipStream_t st, const ApprInst* insts,
+                            const PassItem* items, const uint32_t* n_items, uint32_t zmax);
+hipError_t launch_appr_factor(int dtype, int half, dim3 grid, hipStream_t st, const ApprInst* insts,
+                              const PassItem* items, const uint32_t* n_items);
+hipError_t launch_appr_S(int dtype, const ApprInst* inst, size_t n_fibers, hipStream_t st);
+hipError_t init_appr_attributes();
 void ensure_runtime();  // per-process kernel attributes, once
 
 // ----------------------------------------------------------------- sampler
@@ -225,6 +232,58 @@ class Batch {
     bool partials_valid_ = false;
     int cus_ = 256;
     size_t device_bytes_ = 0;
+};
+
+// ------------------------------------------------- approximate solver
+struct ApprSpec {
+    uint32_t x = 0, y = 0, z = 0;
+    const void* X = nullptr;        // host, x*y
+    const double* alpha = nullptr;  // host, x
+    const double* beta = nullptr;   // host, z
+    const void* nu0 = nullptr;      // host x*z, or null (drawn from seed)
+    const void* mu0 = nullptr;      // host z*y, or null (drawn from seed)
+    uint64_t seed = 0;
+};
+
+// Instances of bld::bld_appr solved together: the factors stay resident,
+// one captured graph per iteration.
+class ApprBatch {
+   public:
+    ApprBatch(int dtype, double eps, const std::vector<ApprSpec>& specs);
+    ~ApprBatch();
+    // restarts from nu0 / mu0 and runs iters iterations
+    void solve_async(size_t iters);
+    void synchronize();
+    // S (x*y*z, formed from the current factors), nu (x*z), mu (z*y); any null
+    void get(size_t idx, void* S, void* nu, void* mu);
+    // mean ms of one iteration (graph replays, HIP events); advances the factors
+    double time_iteration(int reps);
+    size_t device_bytes() const;
+    size_t size() const { return lay_.size(); }
+
+   private:
+    struct L_ {
+        uint32_t x, y, z, ti_rows, n_ti, n_tj, nA, n_cj;
+        size_t off_X, off_nu, off_mu, off_nu0, off_mu0, off_alpha, off_beta, off_psib, off_psia, off_partA,
+            off_psiA, off_pjk, off_pjk2, off_ipk, off_ipk2, off_S;
+    };
+    struct Group {
+        uint32_t zb, zmax, n_items;
+        size_t off_items, off_count;
+    };
+    void enqueue_iteration();
+    void capture();
+
+    int dtype_, psi_ = 0;
+    size_t es_;
+    std::vector<L_> lay_;
+    std::vector<Group> groups_;
+    DeviceBuffer arena_, meta_;
+    size_t off_ai_ = 0, off_bi_ = 0, off_prep_ = 0, off_prep_count_ = 0, off_fnu_ = 0, off_fnu_count_ = 0,
+           off_fmu_ = 0, off_fmu_count_ = 0;
+    uint32_t n_prep_ = 0, n_fnu_ = 0, n_fmu_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipGraphExec_t graph_ = nullptr;
 };
 
 }  // namespace bnmf

@@ -16,6 +16,10 @@ struct bnmf_batch {
     std::unique_ptr<bnmf::Batch> impl;
 };
 
+struct bnmf_appr {
+    std::unique_ptr<bnmf::ApprBatch> impl;
+};
+
 namespace bnmf {
 namespace {
 thread_local std::string g_last_error;
@@ -92,6 +96,27 @@ void solve_one(int dtype, const T* X, size_t x, size_t y, size_t z, const double
     b.get_S(0, S_out);
 }
 
+template <typename T>
+void appr_one(int dtype, const T* X, size_t x, size_t y, size_t z, const double* alpha,
+                     const double* beta, size_t max_iter, double eps, const T* nu0, const T* mu0,
+                     uint64_t seed, T* S_out, T* nu_out, T* mu_out) {
+    check_params(X, x, y, z, alpha, beta);
+    if (x * y == 0) return;
+    bnmf::ApprSpec s;
+    s.x = (uint32_t)x;
+    s.y = (uint32_t)y;
+    s.z = (uint32_t)z;
+    s.X = X;
+    s.alpha = alpha;
+    s.beta = beta;
+    s.nu0 = nu0;
+    s.mu0 = mu0;
+    s.seed = seed;
+    bnmf::ApprBatch a(dtype, eps, {s});
+    a.solve_async(max_iter);
+    a.get(0, S_out, nu_out, mu_out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -144,6 +169,80 @@ bnmf_status bnmf_bld_add_f32(const float* X, size_t x, size_t y, size_t z, const
     return guarded([&] {
         solve_one<float>(BNMF_F32, X, x, y, z, alpha, beta, max_iter, 1, eps, S0, seed, S_out,
                          BNMF_ALGO_ADD);
+    });
+}
+
+bnmf_status bnmf_appr_create(bnmf_appr** out, int dtype, const bnmf_appr_desc* descs, size_t n, double eps) {
+    return guarded([&] {
+        BNMF_REQUIRE(out != nullptr, "out is null");
+        *out = nullptr;
+        check_dtype(dtype);
+        BNMF_REQUIRE(n == 0 || descs != nullptr, "descs is null");
+        std::vector<bnmf::ApprSpec> specs(n);
+        for (size_t t = 0; t < n; ++t) {
+            const bnmf_appr_desc& d = descs[t];
+            if (dtype == BNMF_F32)
+                check_params(static_cast<const float*>(d.X), d.x, d.y, d.z, d.alpha, d.beta);
+            else
+                check_params(static_cast<const double*>(d.X), d.x, d.y, d.z, d.alpha, d.beta);
+            bnmf::ApprSpec& s = specs[t];
+            s.x = (uint32_t)d.x;
+            s.y = (uint32_t)d.y;
+            s.z = (uint32_t)d.z;
+            s.X = d.X;
+            s.alpha = d.alpha;
+            s.beta = d.beta;
+            s.nu0 = d.nu0;
+            s.mu0 = d.mu0;
+            s.seed = d.seed;
+        }
+        auto a = std::make_unique<bnmf_appr>();
+        a->impl = std::make_unique<bnmf::ApprBatch>(dtype, eps, specs);
+        *out = a.release();
+    });
+}
+
+bnmf_status bnmf_appr_solve(bnmf_appr* a, size_t max_iter) {
+    return guarded([&] {
+        BNMF_REQUIRE(a != nullptr, "handle is null");
+        a->impl->solve_async(max_iter);
+        a->impl->synchronize();
+    });
+}
+
+bnmf_status bnmf_appr_get(bnmf_appr* a, size_t idx, void* S_out, void* nu_out, void* mu_out) {
+    return guarded([&] {
+        BNMF_REQUIRE(a != nullptr, "handle is null");
+        a->impl->get(idx, S_out, nu_out, mu_out);
+    });
+}
+
+bnmf_status bnmf_appr_time_iteration(bnmf_appr* a, int reps, double* iter_ms) {
+    return guarded([&] {
+        BNMF_REQUIRE(a != nullptr && iter_ms != nullptr, "null argument");
+        *iter_ms = a->impl->time_iteration(reps);
+    });
+}
+
+void bnmf_appr_destroy(bnmf_appr* a) { delete a; }
+
+bnmf_status bnmf_bld_appr_f64(const double* X, size_t x, size_t y, size_t z, const double* alpha,
+                              const double* beta, size_t max_iter, double eps, const double* nu0,
+                              const double* mu0, uint64_t seed, double* S_out, double* nu_out,
+                              double* mu_out) {
+    return guarded([&] {
+        appr_one<double>(BNMF_F64, X, x, y, z, alpha, beta, max_iter, eps, nu0, mu0, seed, S_out, nu_out,
+                         mu_out);
+    });
+}
+
+bnmf_status bnmf_bld_appr_f32(const float* X, size_t x, size_t y, size_t z, const double* alpha,
+                              const double* beta, size_t max_iter, double eps, const float* nu0,
+                              const float* mu0, uint64_t seed, float* S_out, float* nu_out,
+                              float* mu_out) {
+    return guarded([&] {
+        appr_one<float>(BNMF_F32, X, x, y, z, alpha, beta, max_iter, eps, nu0, mu0, seed, S_out, nu_out,
+                        mu_out);
     });
 }
 

@@ -83,6 +83,46 @@ bnmf_status bnmf_bld_add_f32(const float* X, size_t x, size_t y, size_t z,
                              const double* alpha, const double* beta, size_t max_iter,
                              double eps, const float* S0, uint64_t seed, float* S_out);
 
+/* ------------------------------------------------------------------------
+ * bld::bld_appr (src/bld/bld_appr.hpp:287-372): S kept in factored form
+ * S_ijk = X_ij nu_ik mu_kj / (nu mu + eps)_ij, with multiplicative updates of
+ * nu (x*z, rows sum to 1) and mu (z*y, columns sum to 1) in turn. nu0 / mu0
+ * (NULL: rows of nu and columns of mu drawn from Dirichlet(1_z) with `seed`,
+ * as the reference draws them, :302-318). Outputs S (x*y*z), nu, mu; any may
+ * be NULL. Every digamma is psi_appr (FP64) or the FP32 digamma.
+ * ---------------------------------------------------------------------- */
+bnmf_status bnmf_bld_appr_f64(const double* X, size_t x, size_t y, size_t z,
+                              const double* alpha, const double* beta, size_t max_iter,
+                              double eps, const double* nu0, const double* mu0, uint64_t seed,
+                              double* S_out, double* nu_out, double* mu_out);
+bnmf_status bnmf_bld_appr_f32(const float* X, size_t x, size_t y, size_t z,
+                              const double* alpha, const double* beta, size_t max_iter,
+                              double eps, const float* nu0, const float* mu0, uint64_t seed,
+                              float* S_out, float* nu_out, float* mu_out);
+
+/* Many bld_appr instances resident on the device (no reference counterpart):
+ * the factors stay in HBM, each iteration is a replayed graph. */
+typedef struct bnmf_appr_desc {
+    size_t x, y, z;
+    const void* X;        /* host, x*y, of the dtype */
+    const double* alpha;  /* host, x */
+    const double* beta;   /* host, z */
+    const void* nu0;      /* host x*z or NULL */
+    const void* mu0;      /* host z*y or NULL */
+    uint64_t seed;        /* factor stream when nu0 / mu0 is NULL */
+} bnmf_appr_desc;
+typedef struct bnmf_appr bnmf_appr;
+bnmf_status bnmf_appr_create(bnmf_appr** out, int dtype, const bnmf_appr_desc* descs, size_t n,
+                             double eps);
+/* Restarts every instance from nu0 / mu0 and runs max_iter iterations. */
+bnmf_status bnmf_appr_solve(bnmf_appr* a, size_t max_iter);
+/* S (formed from the current factors), nu, mu of instance idx; any may be NULL. */
+bnmf_status bnmf_appr_get(bnmf_appr* a, size_t idx, void* S_out, void* nu_out, void* mu_out);
+/* Mean milliseconds of one iteration (graph replays timed with HIP events);
+ * advances the factors. */
+bnmf_status bnmf_appr_time_iteration(bnmf_appr* a, int reps, double* iter_ms);
+void bnmf_appr_destroy(bnmf_appr* a);
+
 /* alloc_model::log_marginal_S (src/alloc_model/alloc_model_funcs.hpp:574-586),
  * evaluated on the GPU in FP64. S: x*y*z of the given dtype. */
 bnmf_status bnmf_log_marginal_S(int dtype, const void* S, size_t x, size_t y, size_t z,

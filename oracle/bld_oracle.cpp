@@ -21,6 +21,7 @@
 //     (terms: first :34-77, second :91-132, third :146-169, fourth :179-194)
 //   bld::bld_fact                      src/bld/bld_fact.hpp:36-80
 //   bld::bld_add                       src/bld/bld_add.hpp:69-184
+//   bld::bld_appr                      src/bld/bld_appr.hpp:1-372
 //   util::normalize (L1, along k)      src/util/util.hpp:321-383
 //
 // The reference's exact digamma is GSL's gsl_sf_psi (util_details.hpp:71-73),
@@ -308,6 +309,134 @@ void iterate_add(const double* X, const Dims& d, const double* alpha, const doub
     }
 }
 
+// bld::bld_appr (src/bld/bld_appr.hpp:287-372) state: nu (x*z, rows sum to
+// 1) and mu (z*y, columns sum to 1); S = (X / (nu mu + eps)) nu_ik mu_kj.
+struct Appr {
+    std::vector<double> X_hat, oa, oa2, S, S_pjk, S_ipk, alpha_eph, beta_eph, gp, gm;
+};
+
+// The helper updates of bld_appr.hpp:322-330 (update_X_hat :12-15,
+// update_orig_over_appr :17-23, ..._squared :25-32, update_S :34-48,
+// update_alpha_eph :50-60, update_beta_eph :62-71, update_grad_plus :73-88,
+// update_grad_minus :90-103). Every psi is psi_appr.
+void appr_helpers(const double* X, const Dims& d, const double* alpha, const double* beta,
+                  double eps, const double* nu, const double* mu, Appr& a) {
+    const size_t x = d.x, y = d.y, z = d.z;
+    a.X_hat.assign(x * y, 0.0);
+    a.oa.assign(x * y, 0.0);
+    a.oa2.assign(x * y, 0.0);
+    a.S.assign(x * y * z, 0.0);
+    a.S_pjk.assign(y * z, 0.0);
+    a.S_ipk.assign(x * z, 0.0);
+    a.alpha_eph.assign(x * z, 0.0);
+    a.beta_eph.assign(y * z, 0.0);
+    a.gp.assign(x * y * z, 0.0);
+    a.gm.assign(x * z, 0.0);
+    for (size_t i = 0; i < x; ++i)
+        for (size_t j = 0; j < y; ++j) {
+            double h = 0.0;
+            for (size_t k = 0; k < z; ++k) h += nu[i * z + k] * mu[k * y + j];
+            a.X_hat[i * y + j] = h;
+            a.oa[i * y + j] = X[i * y + j] / (h + eps);
+            a.oa2[i * y + j] = X[i * y + j] / (std::pow(h, 2) + eps);
+        }
+    for (size_t i = 0; i < x; ++i)
+        for (size_t j = 0; j < y; ++j)
+            for (size_t k = 0; k < z; ++k)
+                a.S[d.at(i, j, k)] = a.oa[i * y + j] * nu[i * z + k] * mu[k * y + j];
+    tensor_sums(a.S.data(), d, a.S_pjk.data(), a.S_ipk.data(), nullptr);
+    for (size_t i = 0; i < x; ++i)
+        for (size_t k = 0; k < z; ++k) a.alpha_eph[i * z + k] = a.S_ipk[i * z + k] + alpha[i];
+    for (size_t j = 0; j < y; ++j)
+        for (size_t k = 0; k < z; ++k) a.beta_eph[j * z + k] = a.S_pjk[j * z + k] + beta[k];
+    for (size_t i = 0; i < x; ++i)
+        for (size_t j = 0; j < y; ++j)
+            for (size_t k = 0; k < z; ++k)
+                a.gp[d.at(i, j, k)] = psi_appr(a.beta_eph[j * z + k]) - psi_appr(a.S[d.at(i, j, k)] + 1);
+    std::vector<double> A(z, 0.0);
+    for (size_t i = 0; i < x; ++i)
+        for (size_t k = 0; k < z; ++k) A[k] += a.alpha_eph[i * z + k];
+    for (size_t i = 0; i < x; ++i)
+        for (size_t k = 0; k < z; ++k) a.gm[i * z + k] = psi_appr(A[k]) - psi_appr(a.alpha_eph[i * z + k]);
+}
+
+// update_nu (bld_appr.hpp:105-164): the four accumulations in the
+// reference's loop order, the multiplicative step and the row
+// normalisation nu_i: /= (sum_k nu_ik + eps).
+void appr_update_nu(const Dims& d, double eps, const Appr& a, const double* mu, double* nu,
+                    double* nom_out = nullptr, double* den_out = nullptr) {
+    const size_t x = d.x, y = d.y, z = d.z;
+    std::vector<double> nom(x * z, 0.0), den(x * z, 0.0);
+    for (size_t i = 0; i < x; ++i)
+        for (size_t j = 0; j < y; ++j)
+            for (size_t k = 0; k < z; ++k)
+                nom[i * z + k] += a.oa[i * y + j] * mu[k * y + j] * a.gp[d.at(i, j, k)];
+    for (size_t i = 0; i < x; ++i)
+        for (size_t j = 0; j < y; ++j)
+            for (size_t k = 0; k < z; ++k)
+                for (size_t c = 0; c < z; ++c)
+                    nom[i * z + k] += a.oa2[i * y + j] * mu[k * y + j] * nu[i * z + c] * mu[c * y + j] *
+                                      a.gm[i * z + c];
+    for (size_t i = 0; i < x; ++i)
+        for (size_t j = 0; j < y; ++j)
+            for (size_t k = 0; k < z; ++k)
+                den[i * z + k] += a.oa[i * y + j] * mu[k * y + j] * a.gm[i * z + k];
+    for (size_t i = 0; i < x; ++i)
+        for (size_t j = 0; j < y; ++j)
+            for (size_t k = 0; k < z; ++k)
+                for (size_t c = 0; c < z; ++c)
+                    den[i * z + k] += a.oa2[i * y + j] * mu[k * y + j] * nu[i * z + c] * mu[c * y + j] *
+                                      a.gp[d.at(i, j, c)];
+    if (nom_out) std::memcpy(nom_out, nom.data(), x * z * sizeof(double));
+    if (den_out) std::memcpy(den_out, den.data(), x * z * sizeof(double));
+    for (size_t i = 0; i < x; ++i)
+        for (size_t k = 0; k < z; ++k) nu[i * z + k] *= nom[i * z + k] / (den[i * z + k] + eps);
+    for (size_t i = 0; i < x; ++i) {
+        double rs = 0.0;
+        for (size_t k = 0; k < z; ++k) rs += nu[i * z + k];
+        rs += eps;
+        for (size_t k = 0; k < z; ++k) nu[i * z + k] /= rs;
+    }
+}
+
+// update_mu (bld_appr.hpp:166-226): as update_nu with the roles of i and j
+// exchanged, then the column normalisation mu_:j /= (sum_k mu_kj + eps).
+void appr_update_mu(const Dims& d, double eps, const Appr& a, const double* nu, double* mu,
+                    double* nom_out = nullptr, double* den_out = nullptr) {
+    const size_t x = d.x, y = d.y, z = d.z;
+    std::vector<double> nom(z * y, 0.0), den(z * y, 0.0);
+    for (size_t j = 0; j < y; ++j)
+        for (size_t k = 0; k < z; ++k)
+            for (size_t i = 0; i < x; ++i)
+                nom[k * y + j] += a.oa[i * y + j] * nu[i * z + k] * a.gp[d.at(i, j, k)];
+    for (size_t j = 0; j < y; ++j)
+        for (size_t k = 0; k < z; ++k)
+            for (size_t i = 0; i < x; ++i)
+                for (size_t c = 0; c < z; ++c)
+                    nom[k * y + j] += a.oa2[i * y + j] * nu[i * z + k] * nu[i * z + c] * mu[c * y + j] *
+                                      a.gm[i * z + c];
+    for (size_t j = 0; j < y; ++j)
+        for (size_t k = 0; k < z; ++k)
+            for (size_t i = 0; i < x; ++i)
+                den[k * y + j] += a.oa[i * y + j] * nu[i * z + k] * a.gm[i * z + k];
+    for (size_t j = 0; j < y; ++j)
+        for (size_t k = 0; k < z; ++k)
+            for (size_t i = 0; i < x; ++i)
+                for (size_t c = 0; c < z; ++c)
+                    den[k * y + j] += a.oa2[i * y + j] * nu[i * z + k] * nu[i * z + c] * mu[c * y + j] *
+                                      a.gp[d.at(i, j, c)];
+    if (nom_out) std::memcpy(nom_out, nom.data(), z * y * sizeof(double));
+    if (den_out) std::memcpy(den_out, den.data(), z * y * sizeof(double));
+    for (size_t k = 0; k < z; ++k)
+        for (size_t j = 0; j < y; ++j) mu[k * y + j] *= nom[k * y + j] / (den[k * y + j] + eps);
+    for (size_t j = 0; j < y; ++j) {
+        double cs = 0.0;
+        for (size_t k = 0; k < z; ++k) cs += mu[k * y + j];
+        cs += eps;
+        for (size_t k = 0; k < z; ++k) mu[k * y + j] /= cs;
+    }
+}
+
 void make_work(const Dims& d, Work& w, bool keep_grad_plus) {
     w.S_pjk.assign(d.y * d.z, 0.0);
     w.S_ipk.assign(d.x * d.z, 0.0);
@@ -397,6 +526,56 @@ int oracle_bld_add_step(const double* X, size_t x, size_t y, size_t z, const dou
     make_work(d, w, false);
     std::memcpy(S_out, S_in, x * y * z * sizeof(double));
     iterate_add(X, d, alpha, beta, eps, eph, S_out, w, grad_S);
+    return 0;
+}
+
+// bld::bld_appr (src/bld/bld_appr.hpp:287-372) from explicit nu0 (x*z) and
+// mu0 (z*y) (the reference draws nu's rows and mu's columns from
+// Dirichlet(1) with GSL, :302-318). Writes nu, mu and S = (X/(nu mu + eps))
+// nu_ik mu_kj (:366-369). Returns 0 on success, 1 for a parameter error.
+int oracle_bld_appr(const double* X, size_t x, size_t y, size_t z, const double* alpha,
+                    const double* beta, size_t max_iter, double eps, const double* nu0,
+                    const double* mu0, double* S, double* nu, double* mu) {
+    if (!X || !alpha || !beta || !nu0 || !mu0 || !S || !nu || !mu) return 1;
+    if (check_X(X, x * y)) return 1;
+    const Dims d{x, y, z};
+    std::memcpy(nu, nu0, x * z * sizeof(double));
+    std::memcpy(mu, mu0, z * y * sizeof(double));
+    Appr a;
+    for (size_t eph = 0; eph < max_iter; ++eph) {
+        appr_helpers(X, d, alpha, beta, eps, nu, mu, a);
+        appr_update_nu(d, eps, a, mu, nu);
+        appr_helpers(X, d, alpha, beta, eps, nu, mu, a);
+        appr_update_mu(d, eps, a, nu, mu);
+    }
+    for (size_t i = 0; i < x; ++i)
+        for (size_t j = 0; j < y; ++j) {
+            double h = 0.0;
+            for (size_t k = 0; k < z; ++k) h += nu[i * z + k] * mu[k * y + j];
+            const double oa = X[i * y + j] / (h + eps);
+            for (size_t k = 0; k < z; ++k) S[d.at(i, j, k)] = oa * nu[i * z + k] * mu[k * y + j];
+        }
+    return 0;
+}
+
+// One half-step of bld_appr from (nu, mu): half 0 updates nu (update_nu),
+// half 1 updates mu (update_mu). Writes the updated factor and that
+// update's numerator and denominator (x*z for nu, z*y for mu).
+int oracle_bld_appr_half(const double* X, size_t x, size_t y, size_t z, const double* alpha,
+                         const double* beta, double eps, int half, const double* nu_in,
+                         const double* mu_in, double* out, double* nom, double* den) {
+    if (!X || !alpha || !beta || !nu_in || !mu_in || !out) return 1;
+    if (check_X(X, x * y)) return 1;
+    const Dims d{x, y, z};
+    Appr a;
+    appr_helpers(X, d, alpha, beta, eps, nu_in, mu_in, a);
+    if (half == 0) {
+        std::memcpy(out, nu_in, x * z * sizeof(double));
+        appr_update_nu(d, eps, a, mu_in, out, nom, den);
+    } else {
+        std::memcpy(out, mu_in, z * y * sizeof(double));
+        appr_update_mu(d, eps, a, nu_in, out, nom, den);
+    }
     return 0;
 }
 

@@ -49,6 +49,12 @@ def lib():
         L.oracle_bld_add_step.restype = ctypes.c_int
         L.oracle_bld_add_step.argtypes = [_D, _SZ, _SZ, _SZ, _D, _D, ctypes.c_double, _SZ, _D, _D,
                                           _D]
+        L.oracle_bld_appr.restype = ctypes.c_int
+        L.oracle_bld_appr.argtypes = [_D, _SZ, _SZ, _SZ, _D, _D, _SZ, ctypes.c_double, _D, _D, _D,
+                                      _D, _D]
+        L.oracle_bld_appr_half.restype = ctypes.c_int
+        L.oracle_bld_appr_half.argtypes = [_D, _SZ, _SZ, _SZ, _D, _D, ctypes.c_double, ctypes.c_int,
+                                           _D, _D, _D, _D, _D]
         L.oracle_bld_mult_step.restype = ctypes.c_int
         L.oracle_bld_mult_step.argtypes = [_D, _SZ, _SZ, _SZ, _D, _D, ctypes.c_int, ctypes.c_double,
                                            _D, _D, _D, _D, _D, _D, _D, _D, _D]
@@ -147,6 +153,34 @@ def bld_add_step(X, alpha, beta, S_in, eph=0, eps=1e-50):
     if rc != 0:
         raise ValueError("oracle_bld_add_step: invalid parameters")
     return S, G
+
+
+def bld_appr(X, z, alpha, beta, nu0, mu0, max_iter=1000, eps=1e-50):
+    """bld::bld_appr from nu0 (x, z) and mu0 (z, y); returns (S, nu, mu)."""
+    X, alpha, beta, nu0, mu0 = _f64(X), _f64(alpha), _f64(beta), _f64(nu0), _f64(mu0)
+    x, y = X.shape
+    assert nu0.shape == (x, z) and mu0.shape == (z, y)
+    S, nu, mu = np.empty((x, y, z)), np.empty((x, z)), np.empty((z, y))
+    rc = lib().oracle_bld_appr(_p(X), x, y, z, _p(alpha), _p(beta), int(max_iter), float(eps),
+                               _p(nu0), _p(mu0), _p(S), _p(nu), _p(mu))
+    if rc != 0:
+        raise ValueError("oracle_bld_appr: invalid parameters")
+    return S, nu, mu
+
+
+def bld_appr_half(X, alpha, beta, nu, mu, half, eps=1e-50):
+    """One half-step of bld_appr (half 0: nu, 1: mu); returns the updated
+    factor and that update's (nom, den)."""
+    X, alpha, beta, nu, mu = _f64(X), _f64(alpha), _f64(beta), _f64(nu), _f64(mu)
+    x, y = X.shape
+    z = nu.shape[1]
+    shape = (x, z) if half == 0 else (z, y)
+    out, nom, den = np.empty(shape), np.empty(shape), np.empty(shape)
+    rc = lib().oracle_bld_appr_half(_p(X), x, y, z, _p(alpha), _p(beta), float(eps), int(half),
+                                    _p(nu), _p(mu), _p(out), _p(nom), _p(den))
+    if rc != 0:
+        raise ValueError("oracle_bld_appr_half: invalid parameters")
+    return out, nom, den
 
 
 def log_marginal_S(S, alpha, beta, a, b) -> float:
