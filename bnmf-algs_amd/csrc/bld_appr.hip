@@ -268,21 +268,27 @@ __global__ void __launch_bounds__(64 * WAVES)
 }
 
 // update_nu's step and row normalisation (:155-163) or update_mu's step and
-// column normalisation (:217-225): one thread per row i (HALF 0) or column
-// j (HALF 1); the per-tile partials are added in tile order.
+// column normalisation (:217-225). One wavefront per row i (HALF 0) or
+// column j (HALF 1), lane k: the per-tile partials of the numerator and
+// denominator are added in tile order, the step is applied, and the factor
+// is renormalised by its sum over k (added in k order, as the reference's
+// rowwise / colwise sum). A work item is four rows (columns), one per wave.
 template <typename T, int HALF>
 __global__ void __launch_bounds__(256)
     bld_appr_factor_kernel(const ApprInst* __restrict__ insts, const PassItem* __restrict__ items,
                            const uint32_t* __restrict__ n_items_p) {
+    __shared__ T vbuf[4][64];
+    __shared__ T rsum[4];
     const uint32_t n_items = *n_items_p;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = threadIdx.x >> 6;
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const PassItem item = items[it];
         const ApprInst* __restrict__ I = insts + item.inst;
         const uint32_t x = I->x, y = I->y, z = I->z;
-        const uint32_t r = item.ti * 256u + threadIdx.x;
-        const T epsT = static_cast<T>(I->eps);
         const uint32_t n_rows = HALF == 0 ? x : y;
-        if (r >= n_rows) continue;
+        const uint32_t r = item.ti * 4u + w;
+        const T epsT = static_cast<T>(I->eps);
         // HALF 0: nu[r*z + k], partials part_ipk[(t*x + r)*z + k] over n_tj tiles
         // HALF 1: mu[k*y + r], partials part_pjk[(t*y + r)*z + k] over n_ti tiles
         BNMF_GLOBAL T* __restrict__ F = gmut<T>(HALF == 0 ? I->nu : I->mu);
@@ -290,29 +296,36 @@ __global__ void __launch_bounds__(256)
         const BNMF_GLOBAL T* __restrict__ Pd = gconst<T>(HALF == 0 ? I->part_ipk2 : I->part_pjk2);
         const uint32_t nt = HALF == 0 ? I->n_tj : I->n_ti;
         const size_t tstride = (size_t)n_rows * z;
-        auto fidx = [&](uint32_t k) { return HALF == 0 ? (size_t)r * z + k : (size_t)k * y + r; };
-        SumAcc<T> tot;
-        for (uint32_t k = 0; k < z; ++k) {
+        const bool mine = r < n_rows && lane < z;
+        const size_t fi = HALF == 0 ? (size_t)r * z + lane : (size_t)lane * y + r;
+        if (mine) {
             SumAcc<T> n, d;
+            const size_t base = (size_t)r * z + lane;
             for (uint32_t t = 0; t < nt; ++t) {
-                n.add(Pn[t * tstride + (size_t)r * z + k]);
-                d.add(Pd[t * tstride + (size_t)r * z + k]);
+                n.add(Pn[t * tstride + base]);
+                d.add(Pd[t * tstride + base]);
             }
             T v;
             if constexpr (sizeof(T) == 8) {
-                v = F[fidx(k)] * (n.get() / (d.get() + epsT));
+                v = F[fi] * (n.get() / (d.get() + epsT));
             } else {
                 // eps flushes to zero in float: a row with no data
                 // (numerator 0) keeps a zero factor, as in double
-                const float dn = d.get() + epsT;
-                v = n.get() != 0.0f ? F[fidx(k)] * (n.get() / dn) : 0.0f;
+                v = n.get() != 0.0f ? F[fi] * (n.get() / (d.get() + epsT)) : 0.0f;
             }
-            F[fidx(k)] = v;
-            tot.add(v);
+            vbuf[w][lane] = v;
         }
-        T rs = tot.get() + epsT;
-        if constexpr (sizeof(T) == 4) rs = rs != 0.0f ? rs : 1.0f;
-        for (uint32_t k = 0; k < z; ++k) F[fidx(k)] = F[fidx(k)] / rs;
+        wave_sync();
+        if (r < n_rows && lane == 0) {
+            T rs = T(0);
+            for (uint32_t k = 0; k < z; ++k) rs += vbuf[w][k];
+            rs += epsT;
+            if constexpr (sizeof(T) == 4) rs = rs != 0.0f ? rs : 1.0f;
+            rsum[w] = rs;
+        }
+        wave_sync();
+        if (mine) F[fi] = vbuf[w][lane] / rsum[w];
+        wave_sync();
     }
 }
 

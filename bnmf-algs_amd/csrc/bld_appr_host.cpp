@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -25,10 +26,18 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-// Rows per tile: enough tiles to give the launch ~512 blocks per instance
-// of one tile column, between 4 and 64 rows.
+// Rows per tile: enough tiles to give one instance's launch ~`target`
+// blocks, between 4 and 64 rows. Measured on MI355X (1024x1024x32): 512
+// blocks per iteration beat 2048 and 8192 (more tiles add column partials
+// that the factor and prep kernels must reduce). BNMF_APPR_BLOCKS overrides
+// the target for experiments.
 uint32_t appr_ti_rows(uint32_t x, uint32_t n_tj) {
-    const uint32_t want = std::max<uint32_t>(1, 512 / std::max<uint32_t>(n_tj, 1));
+    static const uint32_t target = [] {
+        const char* e = std::getenv("BNMF_APPR_BLOCKS");
+        const long v = e ? std::strtol(e, nullptr, 10) : 0L;
+        return v > 0 ? (uint32_t)std::min<long>(v, 1 << 20) : 512u;
+    }();
+    const uint32_t want = std::max<uint32_t>(1, target / std::max<uint32_t>(n_tj, 1));
     uint32_t ti = ceil_div(x, want);
     ti = (ti + 3) / 4 * 4;
     return std::min<uint32_t>(std::max<uint32_t>(ti, 4), 64);
@@ -190,8 +199,9 @@ ApprBatch::ApprBatch(int dtype, double eps, const std::vector<ApprSpec>& specs)
         zmax[zb] = std::max(zmax[zb], l.z);
         for (uint32_t c = 0; c < l.n_cj; ++c) prep.push_back(PrepItem{(uint32_t)n, 0, c, 0});
         for (uint32_t c = 0; c < l.nA; ++c) prep.push_back(PrepItem{(uint32_t)n, 1, c, 0});
-        for (uint32_t c = 0; c < ceil_div(l.x, 256); ++c) fnu.push_back(PassItem{(uint32_t)n, c, 0, 0});
-        for (uint32_t c = 0; c < ceil_div(l.y, 256); ++c) fmu.push_back(PassItem{(uint32_t)n, c, 0, 0});
+        // factor updates: four rows (columns) per item, one per wavefront
+        for (uint32_t c = 0; c < ceil_div(l.x, 4); ++c) fnu.push_back(PassItem{(uint32_t)n, c, 0, 0});
+        for (uint32_t c = 0; c < ceil_div(l.y, 4); ++c) fmu.push_back(PassItem{(uint32_t)n, c, 0, 0});
     }
     size_t moff = 0;
     auto mtake = [&](size_t nbytes) {

@@ -4,10 +4,10 @@ tests/test_oracle_bld_appr.py).
 
 Tolerances (per test):
   * FP64 vs the oracle: 1e-12 relative normwise on nu and mu after one
-    iteration, 1e-9 after whole solves (the multiplicative factor updates
-    amplify rounding differences slowly), and the same on S;
-  * FP32 vs the FP64 oracle: 1e-3 relative normwise on S after 100
-    iterations;
+    iteration, 1e-10 after whole solves (measured on MI355X: up to 4e-13
+    after 100 iterations), and the same on S;
+  * FP32 vs the FP64 oracle: 5e-5 relative normwise on S after 100
+    iterations (measured: 3.5e-7 to 4.4e-6);
   * batched vs alone: bitwise.
 """
 import numpy as np
@@ -58,8 +58,8 @@ def test_appr_fp64_solve_matches_oracle(hip, oracle, shape):
     iters = 60 if x * y * z < 200000 else 25
     S, nu, mu = hip.bld_appr(X, z, p, iters, nu0=nu0, mu0=mu0)
     Sr, nur, mur = oracle.bld_appr(X, z, p.alpha, p.beta, nu0, mu0, max_iter=iters)
-    assert _rel(nu, nur) < 1e-9 and _rel(mu, mur) < 1e-9, (_rel(nu, nur), _rel(mu, mur))
-    assert _rel(S, Sr) < 1e-9
+    assert _rel(nu, nur) < 1e-10 and _rel(mu, mur) < 1e-10, (_rel(nu, nur), _rel(mu, mur))
+    assert _rel(S, Sr) < 1e-10
     assert np.abs(S.sum(2) - X).max() <= 1e-12 * max(X.max(), 1)
 
 
@@ -73,7 +73,7 @@ def test_appr_fixture_known_answer(hip, oracle):
     mu0 = rng.dirichlet(np.ones(3), size=10).T.copy()
     S, nu, mu = hip.bld_appr(X, 3, p, nu0=nu0, mu0=mu0)
     Sr, _, _ = oracle.bld_appr(X, 3, p.alpha, p.beta, nu0, mu0)
-    assert _rel(S, Sr) < 1e-9
+    assert _rel(S, Sr) < 1e-10
     assert hip.log_marginal_S(S, p) >= -280
     assert oracle.sparseness(S) >= 0.50
     assert np.linalg.norm(S.sum(2) - X) / np.linalg.norm(X) < 1e-12
@@ -98,7 +98,7 @@ def test_appr_fp32_close_to_fp64_oracle(hip, oracle, shape):
     S, nu, mu = hip.bld_appr(X.astype(np.float32), z, p, 100, nu0=nu0f, mu0=mu0f, dtype=np.float32)
     Sr, _, _ = oracle.bld_appr(X, z, p.alpha, p.beta, nu0f.astype(np.float64),
                                mu0f.astype(np.float64), max_iter=100)
-    assert _rel(S, Sr) < 1e-3, _rel(S, Sr)
+    assert _rel(S, Sr) < 5e-5, _rel(S, Sr)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
