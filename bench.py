@@ -18,6 +18,8 @@ Also measured (same JSON line):
     its normwise error and log-marginal gap against the FP64 form;
   * bld_add_batched_fp32: the sibling additive solver (bld::bld_add) on the
     headline's 256 instances, with the roofline of its pass;
+  * bld_appr_batched_fp32: the sibling approximate solver (bld::bld_appr) on
+    the same instances;
   * roofline of the fused pass kernel of the headline batch (HIP events on
     the batch stream): algorithmic bytes 2*x*y*z*4 + x*y*4 per launch;
   * cpu_baseline: the FP64 C++/OpenMP oracle (a port of the reference CPU
@@ -239,6 +241,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-loop", action="store_true")
     ap.add_argument("--no-add", action="store_true", help="skip the bld_add measurement")
+    ap.add_argument("--no-appr", action="store_true", help="skip the bld_appr measurement")
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="plan the headline batch with the built-in tile rule")
@@ -351,6 +354,25 @@ def main():
             "workload": "the headline's 256 instances, 200 iterations of bld::bld_add"}
         ba.close()
         log(f"[bench] bld_add FP32: {ta*1e3:.1f} ms/solve, pass {pa*1e3:.1f} us ({ach_a:.0f} GB/s)")
+
+    # ----------------------- sibling solver: bld_appr on the same 256 instances
+    if not args.only_headline and not args.no_appr:
+        log("[bench] batched FP32 bld_appr on the same instances")
+        bp = hip.ApprBatch(c3f, dtype=np.float32)
+        bp.solve(2)
+        t0 = time.perf_counter()
+        bp.solve(iters3)
+        tp = time.perf_counter() - t0
+        ip = bp.time_iteration(10)
+        out["bld_appr_batched_fp32"] = {
+            "value": elems3 * iters3 / tp, "unit": "S-element updates/s",
+            "ms_per_solve": tp * 1e3, "iter_us": ip * 1e3,
+            "bound": "latency and VALU (S is formed from the factors in registers; HBM traffic "
+                     "is X and the factors only)",
+            "workload": "the headline's 256 instances, 200 iterations of bld::bld_appr, "
+                        "factors drawn from Dirichlet(1)"}
+        bp.close()
+        log(f"[bench] bld_appr FP32: {tp*1e3:.1f} ms/solve, iteration {ip*1e3:.1f} us")
 
     # --------------------------------------------------------- config 1 (FP64)
     if not args.only_headline:

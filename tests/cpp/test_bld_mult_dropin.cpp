@@ -1,7 +1,7 @@
 // Drop-in checks of the C++ API (bnmf_algs/bnmf_algs.hpp), following the
 // reference's own cases for bld_mult and bld_mult_cuda
 // (test/bld/bld_mult/test_bld_mult.cpp:10-90, test_bld_mult_cuda.cpp:10-90)
-// and for bld_add (test/bld/test_bld_algs.cpp:225-289).
+// and for bld_add and bld_appr (test/bld/test_bld_algs.cpp:225-289, 410-471).
 //
 //   test_bld_mult_dropin cpu   parameter checks that fail before any device work
 //   test_bld_mult_dropin gpu   the full cases, solving on the GPU
@@ -211,6 +211,51 @@ static void bld_add_algorithm_checks() {
     CHECK(err <= 1e-5 * 19);  // 1e-5 relative to max X
 }
 
+// bld::bld_appr's reference cases (test/bld/test_bld_algs.cpp:410-471):
+// zero X gives zero S; on the Experiments.ipynb data, after the default 1000
+// iterations, log_marginal >= -280, sparseness >= 0.50 and S_ij+ = X.
+static void bld_appr_algorithm_checks() {
+    size_t x = 8, y = 10, z = 3;
+    shape<3> tensor_shape{x, y, z};
+    {
+        matrixd X = matrixd::Zero(x, y);
+        alloc_model::Params<double> model_params(tensor_shape);
+        tensord<3> S;
+        matrixd nu, mu;
+        std::tie(S, nu, mu) = bld::bld_appr(X, z, model_params);
+        double lo = 1, hi = -1;
+        for (long t = 0; t < S.size(); ++t) {
+            lo = std::min(lo, S.data()[t]);
+            hi = std::max(hi, S.data()[t]);
+        }
+        CHECK(lo == 0 && hi == 0);
+    }
+    matrixd X(x, y);
+    X << 2., 0., 2., 2., 0., 2., 1., 0., 1., 3., 5., 4., 0., 2., 2., 6., 2., 1., 4., 5., 1., 13.,
+        6., 9., 5., 8., 6., 6., 11., 12., 6., 2., 1., 4., 9., 7., 6., 10., 4., 8., 5., 5., 4., 3.,
+        1., 0., 4., 8., 3., 1., 0., 9., 2., 1., 2., 7., 5., 3., 8., 3., 9., 8., 4., 5., 11., 9., 4.,
+        6., 5., 12., 2., 6., 3., 3., 3., 6., 5., 3., 6., 4.;
+    alloc_model::Params<double> model_params(40, 1, std::vector<double>(x, 1.0),
+                                             std::vector<double>(z, 1.0));
+    tensord<3> S;
+    matrixd nu, mu;
+    std::tie(S, nu, mu) = bld::bld_appr(X, z, model_params);
+    const double lm = alloc_model::log_marginal_S(S, model_params);
+    const double sp = util::sparseness(S);
+    const double gap = sum_gap(X, S);
+    std::printf("bld_appr: log_marginal %.6f sparseness %.6f sum gap %.3e\n", lm, sp, gap);
+    CHECK(lm >= -280);
+    CHECK(sp >= 0.50);
+    CHECK(gap <= 1e-12);
+    // explicit factors: the solve restarts from them
+    tensord<3> S2;
+    matrixd nu2, mu2;
+    std::tie(S2, nu2, mu2) = bld::bld_appr(X, z, model_params, nu, mu, 1);
+    double dn = 0;
+    for (long t = 0; t < nu.size(); ++t) dn = std::max(dn, std::fabs(nu2.data()[t] - nu.data()[t]));
+    CHECK(dn < 1e-3);  // one more step from a converged point moves little
+}
+
 int main(int argc, char** argv) {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     auto mult = [](const matrixd& X, size_t z, const alloc_model::Params<double>& p, size_t it) {
@@ -224,6 +269,10 @@ int main(int argc, char** argv) {
     parameter_checks(mult, gpu);
     parameter_checks(mult_cuda, gpu);
     parameter_checks(add, gpu);
+    auto appr = [](const matrixd& X, size_t z, const alloc_model::Params<double>& p, size_t it) {
+        return std::get<0>(bld::bld_appr(X, z, p, it));
+    };
+    parameter_checks(appr, gpu);
     if (gpu) {
         auto mult0 = [](const matrixd& X, size_t z, const alloc_model::Params<double>& p,
                         auto&&... rest) { return bld::bld_mult(X, z, p, rest...); };
@@ -232,6 +281,7 @@ int main(int argc, char** argv) {
         algorithm_checks(mult0, "bld_mult");
         algorithm_checks(mult0_cuda, "bld_mult_cuda");
         bld_add_algorithm_checks();
+        bld_appr_algorithm_checks();
     }
     std::printf("%d checks, %d failed\n", g_checked, g_failed);
     return g_failed == 0 ? 0 : 1;
