@@ -252,6 +252,8 @@ def main():
                     help="only run 5 device copies of 1 GiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.copy_calibration:
+        # 5 device copies of 1 GiB (the rocprof traffic scale), then the
+        # read + write bandwidth a plain copy reaches, timed with events
         import torch
         n = 256 << 20
         src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -259,7 +261,15 @@ def main():
         for _ in range(5):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 5}), flush=True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 25,
+                          "copy_GBps_read_plus_write": 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
