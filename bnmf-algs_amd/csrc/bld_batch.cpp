@@ -234,10 +234,9 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     arena_ = DeviceBuffer(std::max<size_t>(total, kAlign));
     device_bytes_ = arena_.bytes;
 
-    // uploads (X, alpha, beta, S0). nonneg[n]: S0 >= 0 and every beta_k >= 1,
-    // so grad_plus >= 0 and S stays nonnegative (the kernels then skip the
-    // psi(S+1) guard for S < 0).
-    std::vector<char> tmp;
+    // uploads (X, alpha, beta, S0 or its device draw). nonneg[n]: S0 >= 0
+    // and every beta_k >= 1, so grad_plus >= 0 and S stays nonnegative (the
+    // kernels then skip the psi(S+1) guard for S < 0).
     std::vector<bool> nonneg(specs.size(), false);
     for (size_t n = 0; n < specs.size(); ++n) {
         const InstSpec& s = specs[n];
@@ -249,16 +248,6 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         BNMF_HIP(hipMemcpy(b + L.off_alpha, s.alpha, (size_t)s.x * 8, hipMemcpyHostToDevice));
         BNMF_HIP(hipMemcpy(b + L.off_beta, s.beta, (size_t)s.z * 8, hipMemcpyHostToDevice));
         const void* S0 = s.S0;
-        if (!S0) {
-            tmp.resize(xyz * es_);
-            if (dtype == BNMF_F32)
-                init_S<float>(s.seed, static_cast<const float*>(s.X), s.x, s.y, s.z,
-                              reinterpret_cast<float*>(tmp.data()));
-            else
-                init_S<double>(s.seed, static_cast<const double*>(s.X), s.x, s.y, s.z,
-                               reinterpret_cast<double*>(tmp.data()));
-            S0 = tmp.data();
-        }
         bool nn = true;
         for (uint32_t k = 0; k < s.z; ++k) nn = nn && s.beta[k] >= 1.0;
         if (nn && s.S0) {
@@ -271,8 +260,15 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
             }
         }
         nonneg[n] = nn;
-        BNMF_HIP(hipMemcpy(b + L.off_S0, S0, xyz * es_, hipMemcpyHostToDevice));
-        BNMF_HIP(hipMemcpy(b + L.off_S, S0, xyz * es_, hipMemcpyHostToDevice));
+        if (S0) {
+            BNMF_HIP(hipMemcpy(b + L.off_S0, S0, xyz * es_, hipMemcpyHostToDevice));
+        } else {
+            // init_S drawn on the device from the resident X (the same
+            // Philox stream as the host's init_S, bit for bit): no x*y*z
+            // upload
+            BNMF_HIP(launch_init_s(dtype, b + L.off_S0, b + L.off_X, xy, s.z, s.seed, nullptr));
+        }
+        BNMF_HIP(hipMemcpy(b + L.off_S, b + L.off_S0, xyz * es_, hipMemcpyDeviceToDevice));
     }
 
     // descriptors and work lists of the whole batch (the sums-only and
