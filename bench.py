@@ -208,6 +208,26 @@ def measured_traffic():
         return None
 
 
+def measure_add(hip, c3f, elems3, fibers3, iters3):
+    """bld::bld_add batched FP32 on the headline's instances: solve rate and
+    the roofline of its pass (same algorithmic bytes as bld_mult's)."""
+    log("[bench] batched FP32 bld_add on the same instances")
+    ba = hip.Batch(c3f, dtype=np.float32, algo=hip.ALGO_ADD)
+    ta = timed_solves(ba, iters3, 1, 1)
+    pa, ia = ba.time_pass(20)
+    alg_bytes = 2 * elems3 * 4 + fibers3 * 4
+    ach_a = alg_bytes / (pa * 1e-3) / 1e9
+    ba.close()
+    log(f"[bench] bld_add FP32: {ta*1e3:.1f} ms/solve, pass {pa*1e3:.1f} us ({ach_a:.0f} GB/s)")
+    return {
+        "value": elems3 * iters3 / ta, "unit": "S-element updates/s",
+        "ms_per_solve": ta * 1e3, "pass_us": pa * 1e3, "iter_us": ia * 1e3,
+        "roofline": {"bound": "hbm", "achieved": ach_a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_a / HBM_PEAK_GBS,
+                     "kernel": "bld_pass_f32z_kernel additive step, all launches of one iteration"},
+        "workload": "the headline's 256 instances, 200 iterations of bld::bld_add"}
+
+
 def cpu_baseline(hip):
     from oracle import oracle
     inst = single(hip, 100, 100, 17, 40.0, 0xB1D0)
@@ -242,6 +262,8 @@ def main():
     ap.add_argument("--no-fp64-loop", action="store_true")
     ap.add_argument("--no-add", action="store_true", help="skip the bld_add measurement")
     ap.add_argument("--no-appr", action="store_true", help="skip the bld_appr measurement")
+    ap.add_argument("--add-only", action="store_true",
+                    help="only the batched FP32 bld_add measurement (e.g. for profiling its pass)")
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="plan the headline batch with the built-in tile rule")
@@ -303,6 +325,10 @@ def main():
                  "rows": {f"z{k[0]}f{k[1]}": v for k, v in sorted(res["rows"].items())}}
         if args.save_tune:
             hip.tune_save(args.save_tune)
+    if args.add_only:
+        print(json.dumps({"bld_add_batched_fp32": measure_add(hip, c3f, elems3, fibers3, iters3)}),
+              flush=True)
+        return
     b32 = hip.Batch(c3f, dtype=np.float32, use_psi_appr=True)
     log("[bench] timing batched FP32 solves")
     t_step = timed_solves(b32, iters3, args.steps, args.warmup)
@@ -350,20 +376,7 @@ def main():
 
     # ------------------------ sibling solver: bld_add on the same 256 instances
     if not args.only_headline and not args.no_add:
-        log("[bench] batched FP32 bld_add on the same instances")
-        ba = hip.Batch(c3f, dtype=np.float32, algo=hip.ALGO_ADD)
-        ta = timed_solves(ba, iters3, 1, 1)
-        pa, ia = ba.time_pass(20)
-        ach_a = alg_bytes / (pa * 1e-3) / 1e9
-        out["bld_add_batched_fp32"] = {
-            "value": elems3 * iters3 / ta, "unit": "S-element updates/s",
-            "ms_per_solve": ta * 1e3, "pass_us": pa * 1e3, "iter_us": ia * 1e3,
-            "roofline": {"bound": "hbm", "achieved": ach_a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach_a / HBM_PEAK_GBS,
-                         "kernel": "bld_pass_f32z_kernel additive step, all launches of one iteration"},
-            "workload": "the headline's 256 instances, 200 iterations of bld::bld_add"}
-        ba.close()
-        log(f"[bench] bld_add FP32: {ta*1e3:.1f} ms/solve, pass {pa*1e3:.1f} us ({ach_a:.0f} GB/s)")
+        out["bld_add_batched_fp32"] = measure_add(hip, c3f, elems3, fibers3, iters3)
 
     # ----------------------- sibling solver: bld_appr on the same 256 instances
     if not args.only_headline and not args.no_appr:
