@@ -124,6 +124,8 @@ def lib():
         L.bnmf_appr_time_iteration.restype = st
         L.bnmf_appr_destroy.argtypes = [_VP]
         L.bnmf_appr_destroy.restype = None
+        L.bnmf_batch_solve_tol.argtypes = [_VP, _SZ, ctypes.c_double, _SZ, ctypes.POINTER(_SZ), _D]
+        L.bnmf_batch_solve_tol.restype = st
         for name in ("bnmf_batch_solve", "bnmf_batch_solve_async"):
             getattr(L, name).argtypes = [_VP, _SZ]
             getattr(L, name).restype = st
@@ -545,6 +547,16 @@ class Batch:
 
     def solve_async(self, max_iter: int):
         _check(lib().bnmf_batch_solve_async(self._h, int(max_iter)), "batch_solve_async")
+
+    def solve_tol(self, max_iter: int, tol: float, check_every: int = 10):
+        """Solve from S0 with the FP64 log-marginal convergence check: stops
+        once every instance's relative log-marginal change over check_every
+        iterations is <= tol. Returns (iterations run, log-marginals)."""
+        n = ctypes.c_size_t(0)
+        lm = np.zeros(max(1, len(self.shapes)))
+        _check(lib().bnmf_batch_solve_tol(self._h, int(max_iter), float(tol), int(check_every),
+                                          ctypes.byref(n), _dptr(lm)), "batch_solve_tol")
+        return int(n.value), lm[:len(self.shapes)]
 
     def synchronize(self):
         _check(lib().bnmf_batch_synchronize(self._h), "batch_synchronize")

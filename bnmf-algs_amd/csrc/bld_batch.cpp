@@ -568,6 +568,12 @@ void Batch::solve_async(size_t iters) {
         }
         return;
     }
+    start_solve();
+    replay_iterations(iters - 1);
+    partials_valid_ = true;
+}
+
+void Batch::start_solve() {
     partials_valid_ = false;
     // the additive step size restarts with the solve
     if (algo_ == BNMF_ALGO_ADD)
@@ -575,7 +581,9 @@ void Batch::solve_async(size_t iters) {
     // partial axis sums of S0, then the first iteration reading S0
     enqueue_pass(MODE_SUMS, true);
     enqueue_iteration(true);
-    size_t rem = iters - 1;
+}
+
+void Batch::replay_iterations(size_t rem) {
     if (rem) capture_graphs();
     while (rem >= (size_t)unroll_) {
         BNMF_HIP(hipGraphLaunch(graphU_, stream_));
@@ -585,7 +593,52 @@ void Batch::solve_async(size_t iters) {
         BNMF_HIP(hipGraphLaunch(graph1_, stream_));
         --rem;
     }
-    partials_valid_ = true;
+}
+
+size_t Batch::solve_tol(size_t max_iter, double tol, size_t check_every, double* lm_out) {
+    BNMF_REQUIRE(check_every >= 1, "check_every must be at least 1");
+    BNMF_REQUIRE(tol >= 0.0, "tol must be nonnegative");
+    const size_t n = layouts_.size();
+    if (n_prep_items_ == 0 || max_iter == 0) {
+        solve_async(max_iter);
+        synchronize();
+        if (lm_out) log_marginal(lm_out);
+        return max_iter;
+    }
+    std::vector<double> prev(n, std::nan("")), cur(n);
+    size_t done = 0;
+    bool scored = false;
+    while (done < max_iter) {
+        const size_t chunk = std::min(check_every, max_iter - done);
+        if (done == 0) {
+            start_solve();
+            replay_iterations(chunk - 1);
+        } else {
+            // the scoring pass reused the partial buffers: the partials of
+            // the current S first (the same tiles and order as the step
+            // pass's, so the continued solve is the uninterrupted one)
+            enqueue_pass(MODE_SUMS, false);
+            replay_iterations(chunk);
+        }
+        done += chunk;
+        scored = false;
+        if (done >= max_iter) break;
+        log_marginal(cur.data());
+        scored = true;
+        bool conv = true;
+        for (size_t t = 0; t < n; ++t)
+            conv = conv && std::fabs(cur[t] - prev[t]) <= tol * std::fabs(cur[t]);
+        prev = cur;
+        if (conv) break;
+    }
+    if (!scored) {
+        synchronize();
+        partials_valid_ = true;
+        if (lm_out) log_marginal(lm_out);
+    } else if (lm_out) {
+        std::copy(cur.begin(), cur.end(), lm_out);
+    }
+    return done;
 }
 
 void Batch::synchronize() { BNMF_HIP(hipStreamSynchronize(stream_)); }

@@ -189,8 +189,8 @@ __global__ void __launch_bounds__(64 * WAVES)
 #pragma unroll
                                 for (int v = 0; v < RW; ++v) {
                                     const T sk = vget<T, RW>(sv, v);
-                                    const T sn = sk * ((gp[m * RW + v] + nom) * x_over_s /
-                                                       (vget<T, RW>(gv, v) + den + eps));
+                                    const T sn = as_stored(sk * ((gp[m * RW + v] + nom) * x_over_s /
+                                                                 (vget<T, RW>(gv, v) + den + eps)));
                                     vset<T, RW>(sv, v, sn);
                                     acc[m * RW + v] += ACC(sn);
                                 }
@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(64 * WAVES)
                                     const float sk = vget<T, RW>(sv, v);
                                     const float fct = (gp[m * RW + v] + nom) * x_over_s *
                                                       recip_fast<float>(vget<T, RW>(gv, v) + den_e);
-                                    const float sn = upd ? sk * fct : 0.0f;
+                                    const float sn = as_stored(upd ? sk * fct : 0.0f);
                                     vset<T, RW>(sv, v, sn);
                                     acc[m * RW + v] += ACC(sn);
                                 }
@@ -348,9 +348,9 @@ __global__ void __launch_bounds__(64 * WAVES)
                             for (int v = 0; v < RW; ++v) {
                                 T sn;
                                 if constexpr (sizeof(T) == 8)
-                                    sn = xv * (gz[m * RW + v] / norm);
+                                    sn = as_stored(xv * (gz[m * RW + v] / norm));
                                 else
-                                    sn = gz[m * RW + v] * scale;
+                                    sn = as_stored(gz[m * RW + v] * scale);
                                 vset<T, RW>(sv, v, sn);
                                 acc[m * RW + v] += ACC(sn);
                             }

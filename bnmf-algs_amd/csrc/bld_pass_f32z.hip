@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(64 * WAVES)
                             VR sv;
 #pragma unroll
                             for (int v = 0; v < RW; ++v) {
-                                const float sn = gp[m * RW + v] * scale;
+                                const float sn = as_stored(gp[m * RW + v] * scale);
                                 acc[m * RW + v] += sn;
                                 vset<float, RW>(sv, v, sn);
                             }
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(64 * WAVES)
                                 const int k = 4 * g4 + v;
                                 s4[v] = 0.0f;
                                 if (k < Z) {
-                                    s4[v] = gp[k] * scale;
+                                    s4[v] = as_stored(gp[k] * scale);
                                     acc[k] += s4[v];
                                 }
                             }
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(64 * WAVES)
                     // Phase B: the multiplicative update and the S_+jk partials
                     auto elem_b = [&](int k, float sk, float gmk) {
                         const float r = recip_fast<float>(fmaf(gmk, q, dq));
-                        const float sn = sk * ((gp[k] + nom) * r);
+                        const float sn = as_stored(sk * ((gp[k] + nom) * r));
                         acc[k] += sn;
                         return sn;
                     };

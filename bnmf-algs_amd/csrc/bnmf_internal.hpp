@@ -165,6 +165,12 @@ class Batch {
           int algo = BNMF_ALGO_MULT);
     ~Batch();
     void solve_async(size_t iters);
+    // Restarts from S0 and runs at most max_iter iterations; every
+    // check_every iterations the FP64 log_marginal_S of every instance is
+    // evaluated, and the solve stops once each relative change is <= tol.
+    // Returns the iterations run; lm_out (n values, may be null) receives the
+    // final log-marginals. Synchronous.
+    size_t solve_tol(size_t max_iter, double tol, size_t check_every, double* lm_out);
     void synchronize();
     void get_S(size_t idx, void* host) const;
     void log_marginal(double* out);
@@ -186,6 +192,10 @@ class Batch {
         size_t off_count;
     };
     void enqueue_iteration(bool first) { enqueue_iterations(1, first); }
+    // first iteration of a solve (step counters reset, S0's partials, the
+    // step reading S0), then `rem` more iterations from the captured graphs
+    void start_solve();
+    void replay_iterations(size_t rem);
     // `iters` iterations; each update group runs prep, psi(A) and its pass
     // on its own stream, iteration after iteration, with one fork and join
     // (with_prep = false: the update passes alone, for timing)

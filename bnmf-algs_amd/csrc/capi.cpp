@@ -349,6 +349,15 @@ bnmf_status bnmf_batch_solve_async(bnmf_batch* b, size_t max_iter) {
     });
 }
 
+bnmf_status bnmf_batch_solve_tol(bnmf_batch* b, size_t max_iter, double tol, size_t check_every,
+                                 size_t* iters_run, double* lm_out) {
+    return guarded([&] {
+        BNMF_REQUIRE(b != nullptr, "batch is null");
+        const size_t done = b->impl->solve_tol(max_iter, tol, check_every, lm_out);
+        if (iters_run) *iters_run = done;
+    });
+}
+
 bnmf_status bnmf_batch_synchronize(bnmf_batch* b) {
     return guarded([&] {
         BNMF_REQUIRE(b != nullptr, "batch is null");

@@ -61,6 +61,16 @@ struct SumAcc<double> {
     __device__ __forceinline__ double get() const { return s; }
 };
 
+// A new S element as it is stored: an opaque copy, so that adding it to a
+// partial axis sum is not contracted with the product that produced it (an
+// FMA would add the unrounded product). The partials of a step pass then
+// equal the sums of S as stored, which a sums-only pass reproduces bit for bit.
+template <typename T>
+__device__ __forceinline__ T as_stored(T v) {
+    asm("" : "+v"(v));
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T recip_fast(T v);
 template <>

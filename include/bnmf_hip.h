@@ -172,6 +172,15 @@ bnmf_status bnmf_batch_create_algo(bnmf_batch** out, int dtype, int algo,
 bnmf_status bnmf_batch_solve(bnmf_batch* b, size_t max_iter);
 /* Same, enqueued on the batch's stream without waiting. */
 bnmf_status bnmf_batch_solve_async(bnmf_batch* b, size_t max_iter);
+/* Optional convergence check (no reference counterpart: the reference runs
+ * exactly max_iter iterations): restarts from S0 and runs at most max_iter
+ * iterations; every check_every iterations the FP64 log_marginal_S of every
+ * instance is evaluated and the solve stops once, for each instance,
+ * |lm - lm_previous| <= tol * |lm|. *iters_run receives the iterations run
+ * and lm_out (n values, may be NULL) the final log-marginals. A check costs
+ * about two passes over S. With tol = 0 the result equals bnmf_batch_solve's. */
+bnmf_status bnmf_batch_solve_tol(bnmf_batch* b, size_t max_iter, double tol, size_t check_every,
+                                 size_t* iters_run, double* lm_out);
 bnmf_status bnmf_batch_synchronize(bnmf_batch* b);
 /* Copies instance idx's current S (x*y*z, batch dtype) to host memory. */
 bnmf_status bnmf_batch_get_S(bnmf_batch* b, size_t idx, void* S_out);

@@ -324,3 +324,32 @@ def test_batch_draws_S0_on_device_like_host_init(hip, dtype):
             tol = 1e-13 if dtype == np.float64 else 1e-6
             assert np.allclose(got, ref, rtol=tol, atol=0), shapes[n]
             assert np.allclose(got.sum(2), inst.X, rtol=10 * tol, atol=0)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_convergence_check(hip, oracle, dtype):
+    shapes = [(70, 90, 11), (40, 130, 24), (129, 64, 5)]
+    insts = []
+    for n, (x, y, z) in enumerate(shapes):
+        p, X, S0 = _problem(hip, x, y, z, seed=60 + n, dtype=dtype)
+        insts.append(hip.Instance(X=X, z=z, params=p, S0=S0))
+    with hip.Batch(insts, dtype=dtype) as b:
+        # tol = 0 never stops early, and the checks do not perturb the solve
+        b.solve(95)
+        ref = [b.S(n) for n in range(len(shapes))]
+        lm_ref = b.log_marginal()
+        it, lm = b.solve_tol(95, 0.0, check_every=10)
+        assert it == 95
+        for n in range(len(shapes)):
+            assert np.array_equal(b.S(n), ref[n]), shapes[n]
+        assert np.array_equal(lm, lm_ref)
+        # a loose tolerance stops early, at a multiple of check_every, with
+        # the last two checks within tol of each other
+        it2, lm2 = b.solve_tol(3000, 1e-5, check_every=20)
+        assert it2 < 3000 and it2 % 20 == 0
+        b.solve(it2 - 20)
+        lm_prev = b.log_marginal()
+        assert np.all(np.abs(lm2 - lm_prev) <= 1e-5 * np.abs(lm2))
+        # the log-marginals returned are those of the returned S
+        b.solve(it2)
+        assert np.array_equal(b.log_marginal(), lm2)
