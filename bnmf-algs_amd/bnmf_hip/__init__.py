@@ -145,6 +145,10 @@ def lib():
                                     ctypes.c_double, _D]
         L.bnmf_sample_X.restype = st
         L.bnmf_sample_S.argtypes = L.bnmf_sample_X.argtypes
+        L.bnmf_sample_X_gpu.argtypes = L.bnmf_sample_X.argtypes
+        L.bnmf_sample_X_gpu.restype = st
+        L.bnmf_sample_S_gpu.argtypes = L.bnmf_sample_X.argtypes
+        L.bnmf_sample_S_gpu.restype = st
         L.bnmf_sample_S.restype = st
         L.bnmf_init_S.argtypes = [ctypes.c_int, ctypes.c_uint64, _VP, _SZ, _SZ, _SZ, _VP]
         L.bnmf_init_S.restype = st
@@ -442,6 +446,30 @@ def sample_X(seed: int, shape, model_params: Params) -> np.ndarray:
     _check(lib().bnmf_sample_X(int(seed), x, y, z, _dptr(alpha), _dptr(beta),
                                float(model_params.a), float(model_params.b), _dptr(X)), "sample_X")
     return X
+
+
+def sample_X_gpu(seed: int, shape, model_params: Params) -> np.ndarray:
+    """sample_X with the Poisson draws on the GPU (same streams)."""
+    x, y, z = shape
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    X = np.empty((x, y))
+    _check(lib().bnmf_sample_X_gpu(int(seed), x, y, z, _dptr(alpha), _dptr(beta),
+                                   float(model_params.a), float(model_params.b), _dptr(X)),
+           "sample_X_gpu")
+    return X
+
+
+def sample_S_gpu(seed: int, shape, model_params: Params) -> np.ndarray:
+    """sample_S with the Poisson draws on the GPU (same streams)."""
+    x, y, z = shape
+    alpha = np.ascontiguousarray(model_params.alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(model_params.beta, dtype=np.float64)
+    S = np.empty((x, y, z))
+    _check(lib().bnmf_sample_S_gpu(int(seed), x, y, z, _dptr(alpha), _dptr(beta),
+                                   float(model_params.a), float(model_params.b), _dptr(S)),
+           "sample_S_gpu")
+    return S
 
 
 def sample_S(seed: int, shape, model_params: Params) -> np.ndarray:

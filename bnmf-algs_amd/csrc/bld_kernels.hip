@@ -994,6 +994,38 @@ __global__ void __launch_bounds__(256)
     for (uint32_t k = 0; k < z; ++k) out[k] = (T)(xv * (-log(g.uniform()) / tot));
 }
 
+// Allocation-model draws on the device from the host sampler's Philox
+// streams (alloc_model_funcs.hpp:528-552): mode 0 X_ij ~ Poisson(L_j
+// sum_k W_ik H_kj) (stream ST_X, index i*y + j); mode 1 S_ijk ~ Poisson(W_ik
+// H_kj L_j) (stream ST_S, index (i*y + j)*z + k). One thread per draw.
+__global__ void __launch_bounds__(256)
+    bld_sample_kernel(double* __restrict__ out, const double* __restrict__ W,
+                      const double* __restrict__ H, const double* __restrict__ L, size_t y,
+                      uint32_t z, uint64_t seed, int mode, size_t n) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n) return;
+    if (mode == 0) {
+        const size_t i = idx / y, j = idx - i * y;
+        double lam = 0.0;
+        for (uint32_t k = 0; k < z; ++k) lam += W[i * z + k] * H[(size_t)k * y + j];
+        Philox g(seed, ST_X, idx);
+        out[idx] = poisson_draw(g, lam * L[j]);
+    } else {
+        const size_t f = idx / z, k = idx - f * z;
+        const size_t i = f / y, j = f - i * y;
+        Philox g(seed, ST_S, idx);
+        out[idx] = poisson_draw(g, W[i * z + k] * H[k * y + j] * L[j]);
+    }
+}
+
+hipError_t launch_sample(double* out, const double* W, const double* H, const double* L, size_t y,
+                         uint32_t z, uint64_t seed, int mode, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(bld_sample_kernel, grid, dim3(256), 0, st, out, W, H, L, y, z, seed, mode, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_init_s(int dtype, void* S, const void* X, size_t n_fibers, uint32_t z,
                          uint64_t seed, hipStream_t st) {
     if (n_fibers == 0) return hipSuccess;

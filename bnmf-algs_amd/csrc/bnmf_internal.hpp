@@ -78,6 +78,13 @@ int sample_S(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
              const double* beta, double a, double b, double* S);
 template <typename T>
 int init_S(uint64_t seed, const T* X, size_t x, size_t y, size_t z, T* S);
+// the priors of bnmf_priors: W (x*z), H (z*y), L (y)
+void sample_priors(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                   const double* beta, double a, double b, std::vector<double>& W,
+                   std::vector<double>& H, std::vector<double>& L);
+// device draws of X (mode 0, n = x*y) or S (mode 1, n = x*y*z) from the priors
+hipError_t launch_sample(double* out, const double* W, const double* H, const double* L, size_t y,
+                         uint32_t z, uint64_t seed, int mode, size_t n, hipStream_t st);
 
 // ----------------------------------------------------------- tile planner
 // Largest z handled by the fused pass (fiber registers are sized by bucket).

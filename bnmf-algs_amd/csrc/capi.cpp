@@ -447,6 +447,38 @@ bnmf_status bnmf_sample_S(uint64_t seed, size_t x, size_t y, size_t z, const dou
     });
 }
 
+// sample_X / sample_S with the Poisson draws on the device: the priors
+// (O((x+y)z)) on the host, then one device thread per draw of the same
+// Philox stream as the host sampler.
+static void sample_gpu(int mode, uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                       const double* beta, double a, double b, double* out) {
+    BNMF_REQUIRE(alpha && beta && out, "null argument");
+    BNMF_REQUIRE(a > 0 && b > 0, "a and b must be positive");
+    BNMF_REQUIRE(z >= 1 && z <= 0xffffffffull, "z must be positive");
+    const size_t n = mode == 0 ? x * y : x * y * z;
+    if (n == 0) return;
+    bnmf::ensure_runtime();
+    std::vector<double> W, H, L;
+    bnmf::sample_priors(seed, x, y, z, alpha, beta, a, b, W, H, L);
+    bnmf::DeviceBuffer dW(W.size() * 8), dH(H.size() * 8), dL(L.size() * 8), dout(n * 8);
+    BNMF_HIP(hipMemcpy(dW.ptr, W.data(), W.size() * 8, hipMemcpyHostToDevice));
+    BNMF_HIP(hipMemcpy(dH.ptr, H.data(), H.size() * 8, hipMemcpyHostToDevice));
+    BNMF_HIP(hipMemcpy(dL.ptr, L.data(), L.size() * 8, hipMemcpyHostToDevice));
+    BNMF_HIP(bnmf::launch_sample(dout.as<double>(), dW.as<double>(), dH.as<double>(), dL.as<double>(), y,
+                                 (uint32_t)z, seed, mode, n, nullptr));
+    BNMF_HIP(hipMemcpy(out, dout.ptr, n * 8, hipMemcpyDeviceToHost));
+}
+
+bnmf_status bnmf_sample_X_gpu(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                              const double* beta, double a, double b, double* X_out) {
+    return guarded([&] { sample_gpu(0, seed, x, y, z, alpha, beta, a, b, X_out); });
+}
+
+bnmf_status bnmf_sample_S_gpu(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                              const double* beta, double a, double b, double* S_out) {
+    return guarded([&] { sample_gpu(1, seed, x, y, z, alpha, beta, a, b, S_out); });
+}
+
 bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_t y, size_t z,
                         void* S0_out) {
     return guarded([&] {

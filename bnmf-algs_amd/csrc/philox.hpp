@@ -58,6 +58,41 @@ struct Philox {
     }
 };
 
+// Poisson(mu): Knuth's product method for mu < 12, otherwise the transformed
+// rejection method PTRS (W. Hormann, "The transformed rejection method for
+// generating Poisson random variables", 1993). Host and device run the same
+// operations, so they draw the same value unless their exp / log / lgamma
+// differ in the last bit at a comparison's boundary.
+__host__ __device__ inline double poisson_draw(Philox& g, double mu) {
+    if (!(mu > 0.0)) return 0.0;
+    if (mu < 12.0) {
+        const double lim = exp(-mu);
+        double prod = g.uniform();
+        double k = 0;
+        while (prod > lim) {
+            prod *= g.uniform();
+            k += 1;
+        }
+        return k;
+    }
+    const double smu = sqrt(mu);
+    const double b = 0.931 + 2.53 * smu;
+    const double a = -0.059 + 0.02483 * b;
+    const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    const double log_mu = log(mu);
+    for (;;) {
+        const double U = g.uniform() - 0.5;
+        const double V = g.uniform();
+        const double us = 0.5 - fabs(U);
+        const double k = floor((2.0 * a / us + b) * U + mu + 0.43);
+        if (us >= 0.07 && V <= vr) return k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (log(V) + log(inv_alpha) - log(a / (us * us) + b) <= -mu + k * log_mu - lgamma(k + 1.0))
+            return k;
+    }
+}
+
 // init_S for one fiber (bld_mult_funcs.hpp:36-74): S0_k = X * D_k with
 // D ~ Dirichlet(1_z), i.e. normalised Exp(1) draws. w must hold z doubles.
 template <typename T>

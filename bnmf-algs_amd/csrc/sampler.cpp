@@ -10,7 +10,8 @@
 //   Gamma(a >= 1): Marsaglia & Tsang (2000); a < 1: Gamma(a+1) * U^(1/a).
 //   Normal: Box-Muller.  Dirichlet: normalised Gammas.
 //   Poisson: Knuth's product method for mean < 12, otherwise the
-//   transformed rejection method PTRS (Hormann 1993).
+//   transformed rejection method PTRS (Hormann 1993) (poisson_draw,
+//   philox.hpp, shared with the device sampler).
 // gsl_ran_gamma(r, a, b) takes b as the SCALE; the same convention is kept.
 
 #include <algorithm>
@@ -47,38 +48,6 @@ double gamma_draw(Philox& g, double shape, double scale) {
         const double u = g.uniform();
         if (u < 1.0 - 0.0331 * xn * xn * xn * xn) return scale * d * v;
         if (std::log(u) < 0.5 * xn * xn + d * (1.0 - v + std::log(v))) return scale * d * v;
-    }
-}
-
-double poisson_draw(Philox& g, double mu) {
-    if (!(mu > 0.0)) return 0.0;
-    if (mu < 12.0) {
-        const double lim = std::exp(-mu);
-        double prod = g.uniform();
-        double k = 0;
-        while (prod > lim) {
-            prod *= g.uniform();
-            k += 1;
-        }
-        return k;
-    }
-    // PTRS
-    const double smu = std::sqrt(mu);
-    const double b = 0.931 + 2.53 * smu;
-    const double a = -0.059 + 0.02483 * b;
-    const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4);
-    const double vr = 0.9277 - 3.6224 / (b - 2.0);
-    const double log_mu = std::log(mu);
-    for (;;) {
-        const double U = g.uniform() - 0.5;
-        const double V = g.uniform();
-        const double us = 0.5 - std::fabs(U);
-        const double k = std::floor((2.0 * a / us + b) * U + mu + 0.43);
-        if (us >= 0.07 && V <= vr) return k;
-        if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        if (std::log(V) + std::log(inv_alpha) - std::log(a / (us * us) + b) <=
-            -mu + k * log_mu - std::lgamma(k + 1.0))
-            return k;
     }
 }
 
@@ -135,6 +104,12 @@ void priors(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
 }
 
 }  // namespace
+
+void sample_priors(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                   const double* beta, double a, double b, std::vector<double>& W,
+                   std::vector<double>& H, std::vector<double>& L) {
+    priors(seed, x, y, z, alpha, beta, a, b, W, H, L);
+}
 
 int sample_X(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
              const double* beta, double a, double b, double* X) {

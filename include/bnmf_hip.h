@@ -287,6 +287,14 @@ bnmf_status bnmf_sample_X(uint64_t seed, size_t x, size_t y, size_t z, const dou
 /* The full tensor S (x*y*z) as sample_S draws it. */
 bnmf_status bnmf_sample_S(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
                           const double* beta, double a, double b, double* S_out);
+/* The same draws with the Poisson variates drawn on the GPU (one thread per
+ * variate, the host's Philox streams; the priors W, H, L on the host). Equal
+ * to bnmf_sample_X / bnmf_sample_S unless a device exp / log / lgamma differs
+ * from the host's in the last bit at an acceptance boundary. */
+bnmf_status bnmf_sample_X_gpu(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                              const double* beta, double a, double b, double* X_out);
+bnmf_status bnmf_sample_S_gpu(uint64_t seed, size_t x, size_t y, size_t z, const double* alpha,
+                              const double* beta, double a, double b, double* S_out);
 /* init_S (bld_mult_funcs.hpp:36-74): S0_ijk = X_ij * Dirichlet(1_z)_k. */
 bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_t y, size_t z,
                         void* S0_out);

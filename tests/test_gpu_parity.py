@@ -353,3 +353,20 @@ def test_convergence_check(hip, oracle, dtype):
         # the log-marginals returned are those of the returned S
         b.solve(it2)
         assert np.array_equal(b.log_marginal(), lm2)
+
+
+def test_device_sampler_matches_host_sampler(hip):
+    # the Poisson draws on the GPU from the host sampler's streams: equal
+    # draws (a last-bit difference of exp / log / lgamma at an acceptance
+    # boundary could flip one; none does for these seeds), both regimes of
+    # the draw (means below and above 12)
+    for seed, (x, y, z), a in [(5, (200, 300, 17), 40.0), (6, (64, 90, 8), 400.0)]:
+        p = hip.Params(a, 1.0, [1.0] * x, [1.0] * z)
+        X_h = hip.sample_X(seed, (x, y, z), p)
+        X_d = hip.sample_X_gpu(seed, (x, y, z), p)
+        assert np.array_equal(X_h, X_d)
+    p = hip.Params(40.0, 1.0, [1.0] * 50, [1.0] * 9)
+    S_h = hip.sample_S(11, (50, 60, 9), p)
+    S_d = hip.sample_S_gpu(11, (50, 60, 9), p)
+    assert np.array_equal(S_h, S_d)
+    assert S_d.min() >= 0 and np.all(S_d == np.round(S_d))
