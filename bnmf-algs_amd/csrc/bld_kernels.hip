@@ -34,6 +34,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cstdint>
 
 #include "bld_types.hpp"
@@ -238,7 +239,11 @@ __global__ void __launch_bounds__(64 * WAVES)
                         }
                         const bool upd = live && sumS != 0.0f;
                         const float x_over_s = upd ? xv / (sumS + epsf) : 0.0f;
-                        const float den_e = den + epsf;
+                        // an exact-zero denominator (grad_plus = grad_minus = 0,
+                        // e.g. a single row with beta = 1) would make 0/0; the
+                        // reference's double eps guard makes that update 0
+                        float den_e = den + epsf;
+                        if (den_e == 0.0f) den_e = FLT_MIN;
 #pragma unroll
                         for (int m = 0; m < NCH; ++m) {
                             if ((uint32_t)m < nchunk) {

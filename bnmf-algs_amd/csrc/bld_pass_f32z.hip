@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cstdint>
 
 #include "bld_types.hpp"
@@ -447,7 +448,11 @@ __global__ void __launch_bounds__(64 * WAVES)
                     nom *= xh;
                     den *= xh;
                     const float q = upd ? (sumS + epsf) / xv : 0.0f;
-                    const float dq = upd ? (den + epsf) * q : __builtin_inff();
+                    // den >= 0 here; kept off exact zero so that an all-zero
+                    // numerator (grad_plus = grad_minus = 0, e.g. a single row
+                    // with beta = 1) updates to 0, as the reference's double
+                    // eps guard makes it, instead of 0/0
+                    const float dq = upd ? fmaxf((den + epsf) * q, FLT_MIN) : __builtin_inff();
                     // Phase B: the multiplicative update and the S_+jk partials
                     auto elem_b = [&](int k, float sk, float gmk) {
                         const float r = recip_fast<float>(fmaf(gmk, q, dq));

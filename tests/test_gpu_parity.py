@@ -370,3 +370,25 @@ def test_device_sampler_matches_host_sampler(hip):
     S_d = hip.sample_S_gpu(11, (50, 60, 9), p)
     assert np.array_equal(S_h, S_d)
     assert S_d.min() >= 0 and np.all(S_d == np.round(S_d))
+
+
+@pytest.mark.parametrize("z", [12, 33, 37, 45, 52, 60])
+@pytest.mark.parametrize("x", [1, 3, 5, 9, 131])
+def test_fp32_few_rows_and_large_z(hip, oracle, x, z):
+    # exact-z FP32 kernels at one and several waves per SIMD with row counts
+    # per wave of 0, 1, 2 and many (odd and even), including x = 1, where
+    # grad_plus and grad_minus vanish (beta = 1) and the update is 0/0
+    # without the double eps guard of the reference. Few iterations: an
+    # indexing mistake shows at once, while few-row instances amplify FP32
+    # rounding over many iterations
+    y = 70
+    p, X, S0 = _problem(hip, x, y, z, seed=7 * x + z)
+    S = hip.bld_mult(X.astype(np.float32), z, p, 5, True, S0=S0.astype(np.float32),
+                     dtype=np.float32)
+    ref = oracle.bld_mult(X, z, p.alpha, p.beta, S0.astype(np.float32).astype(np.float64),
+                          max_iter=5, use_psi_appr=True)
+    assert _rel(S, ref) < 1e-4, _rel(S, ref)
+    Sa = hip.bld_add(X.astype(np.float32), z, p, 5, S0=S0.astype(np.float32), dtype=np.float32)
+    refa = oracle.bld_add(X, z, p.alpha, p.beta, S0.astype(np.float32).astype(np.float64),
+                          max_iter=5)
+    assert _rel(Sa, refa) < 1e-4, _rel(Sa, refa)
