@@ -240,11 +240,29 @@ def cpu_baseline(hip):
         times.append(time.perf_counter() - t0)
     med = sorted(times)[1]
     upd = 100 * 100 * 17 * iters
+    # the sibling solvers' oracles on the same input (bld_add's per-fiber
+    # loop is OpenMP-parallel, bld_appr's restatement is sequential)
+    t0 = time.perf_counter()
+    oracle.bld_add(inst.X, inst.z, inst.params.alpha, inst.params.beta, inst.S0, max_iter=200)
+    t_add = time.perf_counter() - t0
+    rng = np.random.default_rng(0xB1D0)
+    nu0 = rng.dirichlet(np.ones(17), size=100)
+    mu0 = rng.dirichlet(np.ones(17), size=100).T.copy()
+    t0 = time.perf_counter()
+    oracle.bld_appr(inst.X, inst.z, inst.params.alpha, inst.params.beta, nu0, mu0, max_iter=50)
+    t_appr = time.perf_counter() - t0
     return {"value": upd / med, "unit": "S-element updates/s", "cores": oracle.num_threads(),
             "kind": "port",
             "sample": "reference CPU config: X 100x100 (allocation model a=40, b=1, alpha=1, "
                       "beta=1), z=17, 1000 iterations, exact digamma; median of 3 runs "
-                      f"({med:.2f} s each); FP64 C++/OpenMP oracle"}
+                      f"({med:.2f} s each); FP64 C++/OpenMP oracle",
+            "siblings": {
+                "bld_add": {"value": 100 * 100 * 17 * 200 / t_add, "unit": "S-element updates/s",
+                            "cores": oracle.num_threads(),
+                            "sample": f"same X and S0, 200 iterations ({t_add:.2f} s)"},
+                "bld_appr": {"value": 100 * 100 * 17 * 50 / t_appr, "unit": "S-element updates/s",
+                             "cores": 1,
+                             "sample": f"same X, Dirichlet(1) factors, 50 iterations ({t_appr:.2f} s)"}}}
 
 
 def main():
