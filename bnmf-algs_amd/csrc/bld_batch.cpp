@@ -66,6 +66,8 @@ uint32_t z_bucket(uint32_t z) {
 }
 
 uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
+    // an empty instance has no tiles (and x = 0 would divide by zero below)
+    if (x == 0 || y == 0) return 4;
     // Row-tile height depends on the instance shape only (never on the rest
     // of a batch), so an instance sums in the same order alone or batched.
     // Tall tiles (up to 128 rows) amortise the tile prologue/epilogue and

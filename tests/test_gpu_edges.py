@@ -68,3 +68,45 @@ def test_edges_bld_appr(hip, oracle, shape):
     S32, _, _ = hip.bld_appr(X.astype(np.float32), z, p, 6, nu0=nu0.astype(np.float32),
                              mu0=mu0.astype(np.float32), dtype=np.float32)
     assert np.all(np.isfinite(S32)) and _rel(S32, Sr) < 1e-4, (shape, _rel(S32, Sr))
+
+
+@pytest.mark.parametrize("shape", [(0, 5, 3), (4, 0, 3), (0, 0, 2)])
+def test_empty_inputs(hip, shape):
+    x, y, z = shape
+    p = hip.Params(1.0, 1.0, [1.0] * x, [1.0] * z)
+    X = np.zeros((x, y))
+    for dt in (np.float64, np.float32):
+        assert hip.bld_mult(X, z, p, 5, dtype=dt).shape == (x, y, z)
+        assert hip.bld_add(X, z, p, 5, dtype=dt).shape == (x, y, z)
+        S, nu, mu = hip.bld_appr(X, z, p, 5, dtype=dt)
+        assert S.shape == (x, y, z) and nu.shape == (x, z) and mu.shape == (z, y)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_empty_instance_in_batches(hip, dtype):
+    # an empty instance among others: the others solve as they do alone
+    pa, Xa, S0a = _case(hip, 20, 30, 5, 8)
+    pe = hip.Params(1.0, 1.0, [], [1.0] * 4)
+    insts = [hip.Instance(X=Xa.astype(dtype), z=5, params=pa, S0=S0a.astype(dtype)),
+             hip.Instance(X=np.zeros((0, 7), dtype=dtype), z=4, params=pe)]
+    alone = hip.bld_mult(Xa.astype(dtype), 5, pa, 10, True, S0=S0a.astype(dtype), dtype=dtype)
+    with hip.Batch(insts, dtype=dtype) as b:
+        b.solve(10)
+        assert np.array_equal(b.S(0), alone)
+        assert b.S(1).shape == (0, 7, 4)
+        lm = b.log_marginal()
+        assert np.isfinite(lm[0])
+
+    class _I:
+        pass
+    ia, ie = _I(), _I()
+    ia.X, ia.z, ia.params, ia.nu0, ia.mu0, ia.seed = Xa.astype(dtype), 5, pa, None, None, 3
+    ie.X, ie.z, ie.params, ie.nu0, ie.mu0, ie.seed = np.zeros((0, 7), dtype=dtype), 4, pe, None, None, 4
+    one = hip.bld_appr(Xa.astype(dtype), 5, pa, 10, seed=3, dtype=dtype)
+    with hip.ApprBatch([ia, ie], dtype=dtype) as b:
+        b.solve(10)
+        got = b.get(0)
+        for g, o in zip(got, one):
+            assert np.array_equal(g, o)
+        S, nu, mu = b.get(1)
+        assert S.shape == (0, 7, 4) and mu.shape == (4, 7)

@@ -63,3 +63,14 @@ def test_save_load_roundtrip(table, tmp_path):
     with pytest.raises(ValueError):
         hip.tune_load(str(bad))
     assert hip.tune_get(np.float32, 30, 1 << 17) == 32   # a bad file changes nothing
+
+
+def test_empty_instances_plan(table):
+    # an empty shape (x or y = 0) has no tiles: the query answers 0 rather
+    # than planning (the batch planner skips such instances itself)
+    hip = table
+    for dt in (np.float32, np.float64):
+        assert hip.tile_rows(dt, 0, 5, 3) == 0
+        assert hip.tile_rows(dt, 7, 0, 3) == 0
+        assert hip.tile_rows(dt, 0, 0, 1) == 0
+        assert hip.tile_rows(dt, 7, 5, 3) >= 1
