@@ -98,6 +98,37 @@ uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
     return std::max<uint32_t>(ti, 4);
 }
 
+// Block b of a launch starts on XCD b % 8: the dispatcher deals workgroups
+// round-robin over the eight XCDs, each with its own L2. The items of a
+// launch (one block each) are laid out so that XCD c runs one contiguous
+// run of the (instance, column tile, row tile) order: the row tiles of a
+// column tile read the same psib slice and the column tiles of a row tile
+// the same grad_minus rows, and then do so through one L2. Item order has
+// no effect on results (every tile writes its own partial-sum slots).
+// BNMF_TILE_ORDER=row keeps the plain row-major order.
+void xcd_order(std::vector<PassItem>& v) {
+    static const bool on = [] {
+        const char* e = std::getenv("BNMF_TILE_ORDER");
+        return !(e && std::strcmp(e, "row") == 0);
+    }();
+    constexpr size_t kXcds = 8;
+    if (!on || v.size() < 2 * kXcds) return;
+    std::stable_sort(v.begin(), v.end(), [](const PassItem& a, const PassItem& b) {
+        if (a.inst != b.inst) return a.inst < b.inst;
+        if (a.tj != b.tj) return a.tj < b.tj;
+        return a.ti < b.ti;
+    });
+    const size_t n = v.size(), q = n / kXcds, r = n % kXcds;
+    std::vector<PassItem> out(n);
+    size_t start = 0;
+    for (size_t c = 0; c < kXcds; ++c) {
+        const size_t cnt = q + (c < r ? 1 : 0);
+        for (size_t s = 0; s < cnt; ++s) out[s * kXcds + c] = v[start + s];
+        start += cnt;
+    }
+    v.swap(out);
+}
+
 InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z, bool with_S0) {
     InstLayout L{};
     const size_t es = dtype == BNMF_F32 ? 4 : 8;
@@ -218,6 +249,14 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         if (dtype == BNMF_F64) psi_ = PSIM_F64_APPR;
     }
     ensure_runtime();
+    // iterations per replayed graph (a power of two up to 64);
+    // BNMF_GRAPH_ITERS overrides it for experiments
+    static const int env_unroll = [] {
+        const char* e = std::getenv("BNMF_GRAPH_ITERS");
+        const long v = e ? std::strtol(e, nullptr, 10) : 0L;
+        return v > 0 ? (int)std::min<long>(v, 64) : 0;
+    }();
+    if (env_unroll) unroll_ = env_unroll;
     int dev = 0;
     BNMF_HIP(hipGetDevice(&dev));
     dev_ = dev;
@@ -304,6 +343,7 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         return (size_t)rows * cols * L.z;
     };
     for (auto& kv : sum_items) {
+        xcd_order(kv.second);
         Bucket bk;
         bk.key = kv.first;
         bk.zmax = 1;
@@ -364,6 +404,7 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
                     for (uint32_t tj = 0; tj < L.n_tj; ++tj) upd_items[gi].push_back(PassItem{local, ti, tj, 0});
             }
             bk.n_items = (uint32_t)upd_items[gi].size();
+            xcd_order(upd_items[gi]);
             ++gi;
         }
     }
@@ -459,8 +500,8 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
 
 Batch::~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    if (graph1_) (void)hipGraphExecDestroy(graph1_);
-    if (graphU_) (void)hipGraphExecDestroy(graphU_);
+    for (hipGraphExec_t g : graphs_)
+        if (g) (void)hipGraphExecDestroy(g);
     for (size_t l = 1; l < side_.size(); ++l) (void)hipStreamDestroy(side_[l]);
     for (hipEvent_t e : fork_ev_) (void)hipEventDestroy(e);
     for (hipEvent_t e : join_ev_) (void)hipEventDestroy(e);
@@ -535,25 +576,22 @@ void Batch::enqueue_iterations(int iters, bool first, bool with_prep) {
     }
 }
 
-void Batch::capture_graphs() {
-    if (graph1_) return;
-    auto capture = [&](int iters, hipGraphExec_t* out) {
-        hipGraph_t g = nullptr;
-        BNMF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        try {
-            enqueue_iterations(iters, false);
-        } catch (...) {
-            (void)hipStreamEndCapture(stream_, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
-        }
-        BNMF_HIP(hipStreamEndCapture(stream_, &g));
-        hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        BNMF_HIP(e);
-    };
-    capture(1, &graph1_);
-    capture(unroll_, &graphU_);
+hipGraphExec_t Batch::graph_of(int level) {
+    if (graphs_[level]) return graphs_[level];
+    hipGraph_t g = nullptr;
+    BNMF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_iterations(1 << level, false);
+    } catch (...) {
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    BNMF_HIP(hipStreamEndCapture(stream_, &g));
+    hipError_t e = hipGraphInstantiate(&graphs_[level], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    BNMF_HIP(e);
+    return graphs_[level];
 }
 
 void Batch::solve_async(size_t iters) {
@@ -586,15 +624,17 @@ void Batch::start_solve() {
 }
 
 void Batch::replay_iterations(size_t rem) {
-    if (rem) capture_graphs();
-    while (rem >= (size_t)unroll_) {
-        BNMF_HIP(hipGraphLaunch(graphU_, stream_));
-        rem -= unroll_;
+    // whole graphs of unroll_ iterations, then the rest in binary: every
+    // graph boundary joins the streams, so a solve has as few as it can
+    int top = 0;
+    while ((2 << top) <= unroll_ && top + 1 < kGraphLevels) ++top;
+    const size_t big = (size_t)1 << top;
+    while (rem >= big) {
+        BNMF_HIP(hipGraphLaunch(graph_of(top), stream_));
+        rem -= big;
     }
-    while (rem > 0) {
-        BNMF_HIP(hipGraphLaunch(graph1_, stream_));
-        --rem;
-    }
+    for (int l = top - 1; l >= 0; --l)
+        if (rem & ((size_t)1 << l)) BNMF_HIP(hipGraphLaunch(graph_of(l), stream_));
 }
 
 size_t Batch::solve_tol(size_t max_iter, double tol, size_t check_every, double* lm_out) {

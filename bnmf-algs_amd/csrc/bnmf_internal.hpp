@@ -91,6 +91,9 @@ hipError_t launch_sample(double* out, const double* W, const double* H, const do
 constexpr uint32_t kMaxZ = 64;
 uint32_t z_bucket(uint32_t z);
 uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z);
+// Lays the items of one pass launch out XCD by XCD (L2 sharing of the
+// per-column psi staging); results do not depend on item order.
+void xcd_order(std::vector<PassItem>& items);
 
 // ------------------------------------------------------------- tile table
 // Rows per pass tile by shape class (tune.cpp); 0 = built-in rule.
@@ -213,7 +216,7 @@ class Batch {
     std::vector<Bucket>& groups_of(int mode) {
         return groups_[(mode == MODE_UPDATE || mode == MODE_ADD) ? 0 : 1];
     }
-    void capture_graphs();
+    hipGraphExec_t graph_of(int level);
 
     int dtype_, psi_;
     int algo_ = BNMF_ALGO_MULT;
@@ -242,8 +245,11 @@ class Batch {
     // when captured)
     std::vector<hipStream_t> side_;
     std::vector<hipEvent_t> fork_ev_, join_ev_;
-    hipGraphExec_t graph1_ = nullptr, graphU_ = nullptr;
-    int unroll_ = 8;
+    // captured iteration graphs of 1, 2, 4, ... unroll_ iterations (level l
+    // holds 2^l), each instantiated on first use
+    static constexpr int kGraphLevels = 7;
+    hipGraphExec_t graphs_[kGraphLevels] = {};
+    int unroll_ = 32;
     // the partial axis sums in the workspace belong to the current S (true
     // after a solve: its last pass accumulates them)
     bool partials_valid_ = false;
