@@ -249,8 +249,8 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         if (dtype == BNMF_F64) psi_ = PSIM_F64_APPR;
     }
     ensure_runtime();
-    // iterations per replayed graph (a power of two up to 64);
-    // BNMF_GRAPH_ITERS overrides it for experiments
+    // iterations per replayed graph, up to 64 and rounded down to a power
+    // of two; BNMF_GRAPH_ITERS overrides the default for experiments
     static const int env_unroll = [] {
         const char* e = std::getenv("BNMF_GRAPH_ITERS");
         const long v = e ? std::strtol(e, nullptr, 10) : 0L;
