@@ -101,7 +101,8 @@ void appr_one(int dtype, const T* X, size_t x, size_t y, size_t z, const double*
                      const double* beta, size_t max_iter, double eps, const T* nu0, const T* mu0,
                      uint64_t seed, T* S_out, T* nu_out, T* mu_out) {
     check_params(X, x, y, z, alpha, beta);
-    if (x * y == 0) return;
+    // an empty X (x*y = 0) runs no iteration; the factors come back as
+    // initialised, as they do for an empty instance of a batch
     bnmf::ApprSpec s;
     s.x = (uint32_t)x;
     s.y = (uint32_t)y;

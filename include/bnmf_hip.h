@@ -89,7 +89,8 @@ bnmf_status bnmf_bld_add_f32(const float* X, size_t x, size_t y, size_t z,
  * nu (x*z, rows sum to 1) and mu (z*y, columns sum to 1) in turn. nu0 / mu0
  * (NULL: rows of nu and columns of mu drawn from Dirichlet(1_z) with `seed`,
  * as the reference draws them, :302-318). Outputs S (x*y*z), nu, mu; any may
- * be NULL. Every digamma is psi_appr (FP64) or the FP32 digamma.
+ * be NULL. Every digamma is psi_appr (FP64) or the FP32 digamma. An empty X
+ * (x*y = 0) runs no iteration: nu and mu are returned as initialised.
  * ---------------------------------------------------------------------- */
 bnmf_status bnmf_bld_appr_f64(const double* X, size_t x, size_t y, size_t z,
                               const double* alpha, const double* beta, size_t max_iter,

@@ -110,3 +110,8 @@ def test_empty_instance_in_batches(hip, dtype):
             assert np.array_equal(g, o)
         S, nu, mu = b.get(1)
         assert S.shape == (0, 7, 4) and mu.shape == (4, 7)
+        # no iteration runs on it: mu stays the seeded Dirichlet draw, alone
+        # and batched
+        _, _, mu_alone = hip.bld_appr(np.zeros((0, 7), dtype=dtype), 4, pe, 10, seed=4, dtype=dtype)
+        assert np.array_equal(mu, mu_alone)
+        assert np.allclose(mu.sum(0), 1.0, atol=1e-6)
