@@ -21,7 +21,9 @@ Also measured (same JSON line):
   * bld_appr_batched_fp32: the sibling approximate solver (bld::bld_appr) on
     the same instances;
   * roofline of the fused pass kernel of the headline batch (HIP events on
-    the batch stream): algorithmic bytes 2*x*y*z*4 + x*y*4 per launch;
+    the batch stream): algorithmic bytes 2*x*y*z*4 + x*y*4 per launch, beside
+    the rate plain streaming kernels reach on the same GPU (stream_ceiling:
+    bnmf-algs_amd/lib/stream_bw, non-temporal copy of 1 GiB);
   * cpu_baseline: the FP64 C++/OpenMP oracle (a port of the reference CPU
     solver, which cannot be built here) on the reference's CPU config
     100x100x17, 1000 iterations, exact digamma.
@@ -39,6 +41,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -208,6 +211,21 @@ def measured_traffic():
         return None
 
 
+def stream_ceiling():
+    """Bandwidth of plain streaming kernels on this GPU (bnmf-algs_amd/lib/
+    stream_bw, run as a child process): the practical read + write ceiling
+    that the update pass, one read and one write of S, is held against."""
+    exe = os.path.join(ROOT, "bnmf-algs_amd", "lib", "stream_bw")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=120, check=True)
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError, IndexError) as e:
+        log(f"[bench] stream_bw failed: {e}")
+        return None
+
+
 def measure_add(hip, c3f, elems3, fibers3, iters3):
     """bld::bld_add batched FP32 on the headline's instances: solve rate and
     the roofline of its pass (same algorithmic bytes as bld_mult's)."""
@@ -288,6 +306,8 @@ def main():
     ap.add_argument("--tune-file", default=None,
                     help="load this tile table instead of autotuning (e.g. for profiling runs)")
     ap.add_argument("--save-tune", default=None, help="save the autotuned tile table here")
+    ap.add_argument("--stream", action="store_true",
+                    help="with --copy-calibration: also run the streaming-kernel ceiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -310,6 +330,8 @@ def main():
         ms = e0.elapsed_time(e1) / 20
         print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 25,
                           "copy_GBps_read_plus_write": 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
+        if args.stream:
+            print(json.dumps({"stream_bw": stream_ceiling()}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -489,6 +511,11 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
+    ceil = stream_ceiling()
+    if ceil:
+        line["roofline"]["stream_copy_nt_GBps"] = ceil["copy_nt_GBps"]
+        line["roofline"]["frac_of_stream_copy"] = achieved / ceil["copy_nt_GBps"]
+        line["stream_ceiling"] = ceil
     line.update(out)
     print(json.dumps(line), flush=True)
 
