@@ -355,6 +355,28 @@ def test_convergence_check(hip, oracle, dtype):
         assert np.array_equal(b.log_marginal(), lm2)
 
 
+@pytest.mark.parametrize("algo", ["mult", "add"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_graph_replays_any_iteration_count(hip, dtype, algo):
+    # a solve replays captured graphs of 32, 16, ..., 1 iterations (the count
+    # in binary); one-iteration chunks through the convergence check give the
+    # same S bit for bit, for counts on both sides of the graph sizes
+    shapes = [(70, 90, 11), (40, 130, 24)]
+    insts = []
+    for n, (x, y, z) in enumerate(shapes):
+        p, X, S0 = _problem(hip, x, y, z, seed=80 + n, dtype=dtype)
+        insts.append(hip.Instance(X=X, z=z, params=p, S0=S0))
+    code = hip.ALGO_ADD if algo == "add" else hip.ALGO_MULT
+    with hip.Batch(insts, dtype=dtype, algo=code) as b:
+        for iters in (1, 7, 31, 33, 64, 95):
+            b.solve(iters)
+            ref = [b.S(n) for n in range(len(shapes))]
+            it, _ = b.solve_tol(iters, 0.0, check_every=1)
+            assert it == iters
+            for n in range(len(shapes)):
+                assert np.array_equal(b.S(n), ref[n]), (iters, shapes[n])
+
+
 def test_device_sampler_matches_host_sampler(hip):
     # the Poisson draws on the GPU from the host sampler's streams: equal
     # draws (a last-bit difference of exp / log / lgamma at an acceptance
