@@ -226,7 +226,9 @@ bnmf_status bnmf_tune_set(int dtype, size_t z, size_t fibers, unsigned rows);
 bnmf_status bnmf_tune_get(int dtype, size_t z, size_t fibers, unsigned* rows);
 bnmf_status bnmf_tune_load(const char* path);
 bnmf_status bnmf_tune_save(const char* path);
-/* Rows per tile an instance of this shape gets now (table or rule). */
+/* Rows per tile an instance of this shape gets now (table or rule); 0 for a
+ * shape with no tiles (x or y = 0) or outside the solver's range (an unknown
+ * dtype, z outside [1, 64], x or y at or above 2^31). */
 unsigned bnmf_tile_rows(int dtype, size_t x, size_t y, size_t z);
 /* Path of the loaded table file, "" if none (valid until the next load). */
 const char* bnmf_tune_source(void);
