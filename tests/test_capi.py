@@ -72,6 +72,33 @@ def test_c_abi_reports_invalid_argument(hip):
     assert st == hip.ERR_INVALID_ARGUMENT
 
 
+def test_sibling_solvers_report_invalid_argument(hip):
+    # bld_add and bld_appr check their parameters as bld_mult does
+    # (check_bld_params, src/bld/util_details.hpp:20-28) before any device work
+    lib = hip.lib()
+    a = np.ones(1)
+    A = a.ctypes.data_as(hip._D)
+    out = np.empty(8)
+    O = out.ctypes.data
+    good = np.array([[1.0, 2.0]])
+    neg = np.array([[1.0, -2.0]])
+    calls = {
+        "add": lambda X, z: lib.bnmf_bld_add_f64(X.ctypes.data, 1, 2, z, A, A, 5, 1e-50, None, 0, O),
+        "appr": lambda X, z: lib.bnmf_bld_appr_f64(X.ctypes.data, 1, 2, z, A, A, 5, 1e-50, None, None,
+                                                   0, O, O, O),
+    }
+    for name, call in calls.items():
+        assert call(neg, 1) == hip.ERR_INVALID_ARGUMENT, name
+        assert b"nonnegative" in lib.bnmf_last_error(), name
+        assert call(good, 0) == hip.ERR_INVALID_ARGUMENT, name
+        assert call(good, 65) == hip.ERR_INVALID_ARGUMENT, name
+    # null X with a nonempty shape
+    st = lib.bnmf_bld_add_f64(None, 1, 2, 1, A, A, 5, 1e-50, None, 0, O)
+    assert st == hip.ERR_INVALID_ARGUMENT and b"X is null" in lib.bnmf_last_error()
+    st = lib.bnmf_bld_appr_f64(None, 1, 2, 1, A, A, 5, 1e-50, None, None, 0, O, O, O)
+    assert st == hip.ERR_INVALID_ARGUMENT and b"X is null" in lib.bnmf_last_error()
+
+
 def test_log_marginal_parameter_checks(hip):
     # test_alloc_model_funcs.cpp:191-209
     S = np.ones((5, 4, 8))
