@@ -13,6 +13,8 @@ Headline workload (one "step" = one full batched solve from S0):
 Also measured (same JSON line):
   * fp64_plain_same_workload: the plain FP64 form (exact digamma) solving the
     same 256 instances one at a time, and the ratio to the batched FP32 rate;
+    fp64_batched_same_workload: the same FP64 form solving the 256 instances
+    as one batch (the ratio against it is reported too);
   * config1_fp64: one 1024x1024x32 instance, 500 iterations, exact digamma;
   * config2_fp32: one 2048x2048x64 instance, 300 iterations, psi_appr, with
     its normwise error and log-marginal gap against the FP64 form;
@@ -31,8 +33,8 @@ Also measured (same JSON line):
   * serving: Poisson arrivals of config-4 requests (x, y log-uniform in
     [64, 2048], z log-uniform in [8, 64], FP32, 200 iterations) into the
     continuous-batching server at half its measured saturation rate, with
-    p50/p99 latency and solves/s beside the one-at-a-time FP64 loop
-    (--requests, default 1000; the full configuration is 10000).
+    p50/p99 latency and solves/s beside the one-at-a-time FP64 loop over the
+    request pool (--requests, default 10000 as the configuration states).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--requests N]
 """
@@ -173,11 +175,11 @@ def serving(hip, n_requests, rate_frac, log, iters=200, pool_size=48):
         wall = time.perf_counter() - t_start
         st = srv.stats()
     lat = np.array(lat)
-    # the plain one-at-a-time FP64 GPU loop on the first requests of the pool
-    n64 = 12
+    # the plain one-at-a-time FP64 GPU loop over the whole request pool
+    n64 = len(pool)
     t0 = time.perf_counter()
     lat64 = []
-    for inst in pool[:n64]:
+    for inst in pool:
         t1 = time.perf_counter()
         hip.bld_mult(inst.X.astype(np.float64), inst.z, inst.params, iters, False,
                      seed=inst.seed, dtype=np.float64)
@@ -202,13 +204,19 @@ def serving(hip, n_requests, rate_frac, log, iters=200, pool_size=48):
 
 def measured_traffic():
     """HBM bytes per pass of the headline workload from the committed rocprof
-    FETCH_SIZE/WRITE_SIZE runs (profiles/bld_pass_traffic.json), or None."""
+    FETCH_SIZE/WRITE_SIZE runs (profiles/bld_pass_traffic.json) and where they
+    come from, or (None, None). The figure is read from the file, not
+    measured in this run: the source names the profile it was taken from."""
     path = os.path.join(ROOT, "profiles", "bld_pass_traffic.json")
     try:
         with open(path) as fh:
-            return float(json.load(fh)["traffic_bytes_per_pass"])
+            d = json.load(fh)
+        src = {"file": "profiles/bld_pass_traffic.json", "profile": d.get("source"),
+               "measured_in_this_run": False,
+               "counters": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, scaled by a 1 GiB copy"}
+        return float(d["traffic_bytes_per_pass"]), src
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 def stream_ceiling():
@@ -244,6 +252,60 @@ def measure_add(hip, c3f, elems3, fibers3, iters3):
                      "frac": ach_a / HBM_PEAK_GBS,
                      "kernel": "bld_pass_f32z_kernel additive step, all launches of one iteration"},
         "workload": "the headline's 256 instances, 200 iterations of bld::bld_add"}
+
+
+def measure_c1(hip, iters=500, solves=1, reps=20):
+    """One 1024x1024x32 instance in the plain FP64 form (exact digamma):
+    solve time and the graph-timed update pass (algorithmic bytes 2*x*y*z*8
+    + x*y*8 per pass)."""
+    log(f"[bench] config 1: 1024x1024x32 FP64 exact, {iters} iterations")
+    c1 = single(hip, 1024, 1024, 32, 40.0, 0xB1D1)
+    b1 = hip.Batch([c1], dtype=np.float64, use_psi_appr=False)
+    t1 = timed_solves(b1, iters, solves, 1 if solves else 0) if solves else float("nan")
+    p1, it1 = b1.time_pass(reps)
+    b1.close()
+    e1 = 1024 * 1024 * 32
+    alg = 2 * e1 * 8 + 1024 * 1024 * 8
+    res = {"value": e1 * iters / t1, "unit": "S-element updates/s", "ms_per_solve": t1 * 1e3,
+           "iterations": iters, "pass_us": p1 * 1e3, "iter_us": it1 * 1e3,
+           "pass_GBps": alg / (p1 * 1e-3) / 1e9, "pass_frac_of_peak": alg / (p1 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "algorithmic_bytes_per_pass": alg}
+    log(f"[bench] config1: {t1*1e3:.1f} ms, pass {p1*1e3:.1f} us ({res['pass_GBps']:.0f} GB/s)")
+    return res
+
+
+def measure_c2(hip, iters=300, reps=20, accuracy=True):
+    """One 2048x2048x64 instance in the FP32-bulk form (psi_appr): solve time,
+    the graph-timed pass and, with accuracy, the normwise error of S and the
+    log-marginal gap against the FP64 form (psi_appr) on the same S0."""
+    log(f"[bench] config 2: 2048x2048x64 FP32 psi_appr, {iters} iterations")
+    c2 = single(hip, 2048, 2048, 64, 40.0, 0xB1D2)
+    b2 = hip.Batch([as_dtype(c2, hip, np.float32)], dtype=np.float32, use_psi_appr=True)
+    t2 = timed_solves(b2, iters, 1, 1)
+    res = {}
+    if accuracy:
+        S2 = b2.S(0)
+        lm2 = b2.log_marginal()[0]
+        b2.solve(iters)
+    p2, it2 = b2.time_pass(reps)
+    b2.close()
+    if accuracy:
+        b2d = hip.Batch([c2], dtype=np.float64, use_psi_appr=True)
+        b2d.solve(iters)
+        S2d = b2d.S(0)
+        lm2d = b2d.log_marginal()[0]
+        b2d.close()
+        res = {"rel_err_S_vs_fp64": rel_err(S2, S2d),
+               "log_marginal_gap_vs_fp64": abs(lm2 - lm2d) / abs(lm2d)}
+        del S2, S2d
+    e2 = 2048 * 2048 * 64
+    alg = 2 * e2 * 4 + 2048 * 2048 * 4
+    res = {"value": e2 * iters / t2, "unit": "S-element updates/s", "ms_per_solve": t2 * 1e3,
+           "iterations": iters, "pass_us": p2 * 1e3, "iter_us": it2 * 1e3,
+           "pass_GBps": alg / (p2 * 1e-3) / 1e9, "pass_frac_of_peak": alg / (p2 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           **res}
+    log(f"[bench] config2: {t2*1e3:.1f} ms, pass {p2*1e3:.1f} us ({res['pass_GBps']:.0f} GB/s)")
+    return res
 
 
 def cpu_baseline(hip):
@@ -290,8 +352,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--serving", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--requests", type=int, default=1000,
-                    help="serving requests (the full configuration is 10000)")
+    ap.add_argument("--requests", type=int, default=10000,
+                    help="serving requests (the configuration's 10000 by default)")
     ap.add_argument("--rate-frac", type=float, default=0.5,
                     help="serving arrival rate as a fraction of measured saturation")
     ap.add_argument("--no-cpu", action="store_true")
@@ -301,6 +363,11 @@ def main():
     ap.add_argument("--add-only", action="store_true",
                     help="only the batched FP32 bld_add measurement (e.g. for profiling its pass)")
     ap.add_argument("--only-headline", action="store_true")
+    ap.add_argument("--only", choices=("c1", "c2"), default=None,
+                    help="only one single-instance configuration (e.g. for profiling its pass): "
+                         "c1 = 1024x1024x32 FP64 exact, c2 = 2048x2048x64 FP32")
+    ap.add_argument("--iters", type=int, default=None,
+                    help="with --only: iterations of the solve (default: the configuration's)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="plan the headline batch with the built-in tile rule")
     ap.add_argument("--tune-file", default=None,
@@ -341,6 +408,13 @@ def main():
     import bnmf_hip as hip
     hip.init(0)
     out = {}
+    if args.only == "c1":
+        print(json.dumps({"config1_fp64": measure_c1(hip, iters=args.iters or 500)}), flush=True)
+        return
+    if args.only == "c2":
+        print(json.dumps({"config2_fp32": measure_c2(hip, iters=args.iters or 300, accuracy=False)}),
+              flush=True)
+        return
 
     # ----------------------------------------------------- headline: config 3
     log("[bench] building 256-instance batch")
@@ -413,6 +487,20 @@ def main():
                                    "instances": len(errs)}
         log(f"[bench] FP64 plain loop: {t64:.2f} s, {fp64['value']:.3e} upd/s; "
             f"max rel err {max(errs):.2e}, max lm gap {gaps.max():.2e}")
+        # the same FP64 form with the 256 instances in one batch: the ratio
+        # against a batched FP64 solve, not only against per-call launches
+        log("[bench] FP64 batch of the same 256 instances")
+        b64 = hip.Batch(c3d, dtype=np.float64, use_psi_appr=False)
+        t64b = timed_solves(b64, iters3, 1, 1)
+        p64, i64 = b64.time_pass(10)
+        b64.close()
+        out["fp64_batched_same_workload"] = {
+            "value": elems3 * iters3 / t64b, "unit": "S-element updates/s",
+            "ms_per_solve": t64b * 1e3, "pass_us": p64 * 1e3, "iter_us": i64 * 1e3,
+            "pass_GBps": (2 * elems3 * 8 + fibers3 * 8) / (p64 * 1e-3) / 1e9,
+            "dtype": "f64", "digamma": "exact", "mode": "the 256 instances as one batch"}
+        out["ratio_vs_fp64_batched"] = value / out["fp64_batched_same_workload"]["value"]
+        log(f"[bench] FP64 batch: {t64b*1e3:.1f} ms/solve, pass {p64*1e3:.1f} us")
 
     # ------------------------ sibling solver: bld_add on the same 256 instances
     if not args.only_headline and not args.no_add:
@@ -437,45 +525,10 @@ def main():
         bp.close()
         log(f"[bench] bld_appr FP32: {tp*1e3:.1f} ms/solve, iteration {ip*1e3:.1f} us")
 
-    # --------------------------------------------------------- config 1 (FP64)
+    # ------------------------------------------- configs 1 (FP64) and 2 (FP32)
     if not args.only_headline:
-        log("[bench] config 1: 1024x1024x32 FP64 exact, 500 iterations")
-        c1 = single(hip, 1024, 1024, 32, 40.0, 0xB1D1)
-        b1 = hip.Batch([c1], dtype=np.float64, use_psi_appr=False)
-        t1 = timed_solves(b1, 500, 1, 1)
-        p1, it1 = b1.time_pass(20)
-        e1 = 1024 * 1024 * 32
-        out["config1_fp64"] = {
-            "value": e1 * 500 / t1, "unit": "S-element updates/s", "ms_per_solve": t1 * 1e3,
-            "pass_us": p1 * 1e3, "iter_us": it1 * 1e3,
-            "pass_GBps": (2 * e1 * 8 + 1024 * 1024 * 8) / (p1 * 1e-3) / 1e9}
-        b1.close()
-        log(f"[bench] config1: {t1*1e3:.1f} ms, pass {p1*1e3:.1f} us")
-
-        log("[bench] config 2: 2048x2048x64 FP32 psi_appr, 300 iterations")
-        c2 = single(hip, 2048, 2048, 64, 40.0, 0xB1D2)
-        b2 = hip.Batch([as_dtype(c2, hip, np.float32)], dtype=np.float32, use_psi_appr=True)
-        t2 = timed_solves(b2, 300, 1, 1)
-        S2 = b2.S(0)
-        lm2 = b2.log_marginal()[0]
-        b2.solve(300)
-        p2, it2 = b2.time_pass(20)
-        b2.close()
-        b2d = hip.Batch([c2], dtype=np.float64, use_psi_appr=True)
-        b2d.solve(300)
-        S2d = b2d.S(0)
-        lm2d = b2d.log_marginal()[0]
-        b2d.close()
-        e2 = 2048 * 2048 * 64
-        out["config2_fp32"] = {
-            "value": e2 * 300 / t2, "unit": "S-element updates/s", "ms_per_solve": t2 * 1e3,
-            "pass_us": p2 * 1e3, "iter_us": it2 * 1e3,
-            "pass_GBps": (2 * e2 * 4 + 2048 * 2048 * 4) / (p2 * 1e-3) / 1e9,
-            "rel_err_S_vs_fp64": rel_err(S2, S2d),
-            "log_marginal_gap_vs_fp64": abs(lm2 - lm2d) / abs(lm2d)}
-        del S2, S2d
-        log(f"[bench] config2: {t2*1e3:.1f} ms, pass {p2*1e3:.1f} us, "
-            f"err {out['config2_fp32']['rel_err_S_vs_fp64']:.2e}")
+        out["config1_fp64"] = measure_c1(hip)
+        out["config2_fp32"] = measure_c2(hip)
 
     if not args.no_serving and (args.serving or not args.only_headline):
         out["serving"] = serving(hip, args.requests, args.rate_frac, log)
@@ -485,6 +538,7 @@ def main():
         log("[bench] CPU baseline (FP64 oracle, reference CPU config)")
         cpu = cpu_baseline(hip)
 
+    traffic, traffic_src = measured_traffic()
     line = {
         "metric": "bld_mult S-elem updates/s (FP32 batched vs FP64 plain) + p50/p99 solve latency",
         "value": value,
@@ -505,7 +559,8 @@ def main():
                    "pass_items": info["pass_items"], "pass_launches": info["n_buckets"],
                    "tile_autotune": tuned, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "bld_pass_f32z_kernel (FP32 update), all launches of one iteration",
                      "pass_us": pass_ms * 1e3, "iter_us": iter_ms * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes},

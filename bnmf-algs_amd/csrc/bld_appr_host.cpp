@@ -52,7 +52,9 @@ ApprBatch::ApprBatch(int dtype, double eps, const std::vector<ApprSpec>& specs)
     // layouts
     size_t total = 0;
     for (const ApprSpec& s : specs) {
-        BNMF_REQUIRE(s.z >= 1 && s.z <= kMaxZ, "z must be in [1, 64]");
+        // the approximate solver's passes keep a tile's factors per column
+        // in LDS and registers sized for fibers of at most 64
+        BNMF_REQUIRE(s.z >= 1 && s.z <= kFusedZ, "bld_appr supports z in [1, 64]");
         L_ l{};
         l.x = s.x;
         l.y = s.y;

@@ -35,6 +35,7 @@ void ensure_runtime() {
     std::call_once(once, [] {
         status = init_kernel_attributes();
         if (status == hipSuccess) status = init_f32z_attributes();
+        if (status == hipSuccess) status = init_f64_attributes();
         if (status == hipSuccess) status = init_appr_attributes();
     });
     BNMF_HIP(status);
@@ -59,22 +60,32 @@ DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
 
 // ---------------------------------------------------------------- planner
 uint32_t z_bucket(uint32_t z) {
+    if (z > kFusedZ) return 0;
     if (z <= 8) return 8;
     if (z <= 16) return 16;
     if (z <= 32) return 32;
     return 64;
 }
 
+uint32_t tile_cols_of(int dtype, uint32_t z) {
+    if (z > kFusedZ) return kWideCols;
+    if (dtype == BNMF_F64) return z <= 32 ? kTileColsF64 : kTileColsF64 / 2;
+    return kTileCols;
+}
+
 uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
     // an empty instance has no tiles (and x = 0 would divide by zero below)
     if (x == 0 || y == 0) return 4;
+    // deep fibers: the wide pass's fixed tiles
+    if (z > kFusedZ) return kWideRows;
     // Row-tile height depends on the instance shape only (never on the rest
     // of a batch), so an instance sums in the same order alone or batched.
     // Tall tiles (up to 128 rows) amortise the tile prologue/epilogue and
     // cut the S_+jk partial traffic to 1/TI of S; an instance with a million
     // fibers or more takes shorter tiles if that is what it needs to give
-    // 256 CUs two tiles each when it runs alone.
-    const uint32_t n_tj = (y + kTileCols - 1) / kTileCols;
+    // 256 CUs two tiles each when it runs alone (FP64, whose step is bound by
+    // arithmetic rather than bytes, four).
+    const uint32_t n_tj = (y + tile_cols_of(dtype, z) - 1) / tile_cols_of(dtype, z);
     // Measured on MI355X: small instances (batched) prefer 64-row tiles,
     // instances of a million fibers or more 128-row tiles. BNMF_TI_CAP
     // overrides the cap for experiments.
@@ -89,7 +100,8 @@ uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z) {
                                      : tuned ? tuned : (large ? 128u : 64u);
     uint32_t nt = (x + cap - 1) / cap;
     if (large) {
-        const uint32_t fill = (512 + n_tj - 1) / n_tj;
+        const uint32_t target = dtype == BNMF_F64 ? 1024u : 512u;
+        const uint32_t fill = (target + n_tj - 1) / n_tj;
         const uint32_t most = (x + 15) / 16;
         nt = std::max(nt, std::min(fill, most));
     }
@@ -138,8 +150,9 @@ InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z, bool wit
     L.zb = z_bucket(z);
     read_layout(dtype, z, &L.rw, &L.hs, &L.hm);
     L.ti_rows = choose_ti_rows(dtype, x, y, z);
+    L.tile_cols = tile_cols_of(dtype, z);
     L.n_ti = ceil_div(x, L.ti_rows);
-    L.n_tj = ceil_div(y, kTileCols);
+    L.n_tj = ceil_div(y, L.tile_cols);
     L.nA = ceil_div(x, kPrepRows);
     L.n_cj = ceil_div(y, kPrepCols);
     size_t off = 0;
@@ -197,6 +210,7 @@ BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double 
     d.hs = L.hs;
     d.hm = L.hm;
     d.ti_rows = L.ti_rows;
+    d.tile_cols = L.tile_cols;
     d.n_ti = L.n_ti;
     d.n_tj = L.n_tj;
     d.nA = L.nA;
@@ -213,13 +227,23 @@ BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double 
 }
 
 PassKey pass_key(int dtype, int mode, const InstLayout& L, bool nonneg) {
-    // BNMF_PASS_GENERIC=1 routes the FP32 update through the generic
-    // z-bucket kernel (same arithmetic; for comparisons)
+    // BNMF_PASS_GENERIC=1 routes the FP32 steps through the generic z-bucket
+    // kernel, for comparisons. Its update is the same formula evaluated in
+    // another order (the exact-z kernel folds X_reciprocal and X / S_ij+
+    // into per-fiber constants), so the two agree to FP32 rounding, not
+    // bit for bit.
     static const bool generic = [] {
         const char* e = std::getenv("BNMF_PASS_GENERIC");
         return e != nullptr && e[0] == '1';
     }();
     PassKey k;
+    if (L.z > kFusedZ) return k;  // the wide pass (zexact = zb = 0)
+    if (dtype == BNMF_F64 && (mode == MODE_UPDATE || mode == MODE_ADD)) {
+        k.f64 = 1;
+        k.zb = (int)L.zb;
+        k.rw = (int)L.rw;
+        return k;
+    }
     if (dtype == BNMF_F32 && (mode == MODE_UPDATE || mode == MODE_ADD) && nonneg && !generic) {
         k.zexact = (int)L.z;
         return k;
@@ -232,6 +256,11 @@ PassKey pass_key(int dtype, int mode, const InstLayout& L, bool nonneg) {
 hipError_t launch_pass_key(const PassKey& k, int dtype, int psi, int mode, dim3 grid,
                            hipStream_t st, const BldInst* insts, const PassItem* items,
                            const uint32_t* n_items, uint32_t zmax) {
+    if (k.wide()) return launch_pass_wide(dtype, psi, mode, grid, st, insts, items, n_items);
+    if (k.f64) {
+        if (dtype != BNMF_F64 || (mode != MODE_UPDATE && mode != MODE_ADD)) return hipErrorInvalidValue;
+        return launch_pass_f64(psi, mode, k.zb, k.rw, grid, st, insts, items, n_items);
+    }
     if (k.zexact > 0) {
         if (dtype != BNMF_F32 || (mode != MODE_UPDATE && mode != MODE_ADD)) return hipErrorInvalidValue;
         return launch_pass_f32z(k.zexact, mode, grid, st, insts, items, n_items);
@@ -267,7 +296,8 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     size_t total = 0;
     for (size_t n = 0; n < specs.size(); ++n) {
         const InstSpec& s = specs[n];
-        BNMF_REQUIRE(s.z >= 1 && s.z <= kMaxZ, "z must be in [1, 64]");
+        BNMF_REQUIRE(s.x < (1u << 31) && s.y < (1u << 31) && s.z < (1u << 31),
+                     "x, y and z must be below 2^31");
         layouts_.push_back(plan_instance(dtype, s.x, s.y, s.z));
         base_off[n] = total;
         total += layouts_.back().bytes;
@@ -339,7 +369,7 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
     }
     auto tile_work = [&](const InstLayout& L, uint32_t ti, uint32_t tj) {
         const uint32_t rows = std::min(L.ti_rows, L.x - ti * L.ti_rows);
-        const uint32_t cols = std::min(kTileCols, L.y - tj * kTileCols);
+        const uint32_t cols = std::min(L.tile_cols, L.y - tj * L.tile_cols);
         return (size_t)rows * cols * L.z;
     };
     for (auto& kv : sum_items) {

@@ -68,8 +68,8 @@ __global__ void __launch_bounds__(64 * WAVES)
         const uint32_t x = I->x, y = I->y, z = I->z;
         const uint32_t i0 = item.ti * I->ti_rows;
         const uint32_t i1 = min(x, i0 + I->ti_rows);
-        const uint32_t j0 = item.tj * kTileCols;
-        const uint32_t nj = min(kTileCols, y - j0);
+        const uint32_t j0 = item.tj * I->tile_cols;
+        const uint32_t nj = min(I->tile_cols, y - j0);
         const uint32_t n_el = nj * z;
         const uint32_t nchunk = z / RW;                                 // fiber chunks in use
         const uint32_t hsw = RW > 1 ? ((lane >> I->hs) & I->hm) : 0u;  // visit order
@@ -517,7 +517,7 @@ __global__ void __launch_bounds__(256)
             const size_t stride = (size_t)y * z;
             for (uint32_t e = tid; e < nc * z; e += 256) {
                 const size_t idx = (size_t)c0 * z + e;
-                const uint32_t k = e - fiber_of(e, inv_z) * z;
+                const uint32_t k = z <= kFusedZ ? e - fiber_of(e, inv_z) * z : e % z;
                 SumAcc<T> s;
                 add_strided(s, pp + idx, stride, I->n_ti);
                 if constexpr (RAW) {
@@ -526,6 +526,30 @@ __global__ void __launch_bounds__(256)
                     const T beta_eph = s.get() + static_cast<T>(I->beta[k]);
                     psib[idx] = psi_sel<PSI, T>(beta_eph);
                 }
+            }
+        } else if (z > kFusedZ) {
+            // deep fibers: one thread per k walks the chunk's rows in order,
+            // so the alpha_eph column sums need no LDS staging
+            const uint32_t r0 = item.chunk * kPrepRows;
+            const uint32_t nr = min(kPrepRows, x - r0);
+            const T* __restrict__ pp = static_cast<const T*>(I->part_ipk);
+            T* __restrict__ psia = static_cast<T*>(I->psia);
+            const size_t stride = (size_t)x * z;
+            for (uint32_t k = tid; k < z; k += 256) {
+                SumAcc<T> A;
+                for (uint32_t r = 0; r < nr; ++r) {
+                    const size_t idx = (size_t)(r0 + r) * z + k;
+                    SumAcc<T> s;
+                    add_strided(s, pp + idx, stride, I->n_tj);
+                    if constexpr (RAW) {
+                        psia[idx] = s.get();
+                    } else {
+                        const T alpha_eph = s.get() + static_cast<T>(I->alpha[r0 + r]);
+                        psia[idx] = psi_sel<PSI, T>(alpha_eph);
+                        A.add(alpha_eph);
+                    }
+                }
+                if constexpr (!RAW) static_cast<T*>(I->partA)[(size_t)item.chunk * z + k] = A.get();
             }
         } else {
             const uint32_t r0 = item.chunk * kPrepRows;
@@ -569,7 +593,18 @@ __global__ void __launch_bounds__(256)
     for (uint32_t n = blockIdx.x; n < n_inst; n += gridDim.x) {
         const BldInst* __restrict__ I = insts + n;
         const uint32_t z = I->z, nA = I->nA;
-        if ((size_t)I->x * I->y == 0) continue;
+        if ((size_t)I->x * I->y * z == 0) continue;
+        if (z > kFusedZ) {
+            // deep fibers: one thread per k sums the chunk sums in order
+            for (uint32_t k = tid; k < z; k += 256) {
+                SumAcc<T> B;
+                add_strided(B, static_cast<const T*>(I->partA) + k, z, nA);
+                gmut<T>(I->psiA)[k] = psi_sel<PSI, T>(B.get());
+            }
+            if (tid == 0 && I->iter != nullptr) I->iter[0] += 1u;
+            __syncthreads();
+            continue;
+        }
         const uint32_t P = 256u / z;  // partial sums per k (z <= 64: P >= 4)
         const uint32_t k = tid % z, p = tid / z;
         const BNMF_GLOBAL T* __restrict__ pA = gconst<T>(I->partA);
@@ -646,6 +681,27 @@ __global__ void __launch_bounds__(256)
             }
             const double s = block_sum_256(v, red);
             if (tid == 0) I->score[ntile + I->nA + item.chunk] = s;
+        } else if (z > kFusedZ) {
+            // the same terms for deep fibers: one thread per k over the
+            // chunk's rows (no LDS staging of S_i+k)
+            const uint32_t r0 = item.chunk * kPrepRows;
+            const uint32_t nr = min(kPrepRows, x - r0);
+            const double* __restrict__ pp = static_cast<const double*>(I->part_ipk);
+            const size_t stride = (size_t)x * z;
+            double v = 0.0;
+            for (uint32_t k = tid; k < z; k += 256) {
+                double cs = 0.0;
+                for (uint32_t r = 0; r < nr; ++r) {
+                    const size_t idx = (size_t)(r0 + r) * z + k;
+                    double s = 0.0;
+                    for (uint32_t t = 0; t < I->n_tj; ++t) s += pp[t * stride + idx];
+                    v += lgamma(I->alpha[r0 + r] + s);
+                    cs += s;
+                }
+                I->score[ntile + I->nA + I->n_cj + (size_t)item.chunk * z + k] = cs;
+            }
+            const double s = block_sum_256(v, red);
+            if (tid == 0) I->score[ntile + item.chunk] = s;
         } else {
             // first term for a chunk of rows i: sum lgamma(alpha_i + S_i+k)
             // and the chunk's column sums of S_i+k (for S_++k)
@@ -917,6 +973,8 @@ __global__ void __launch_bounds__(256)
         r = (double)psi_sp1<PSI_F32, float>((float)v - 1.0f);
     } else if (kind == 1) {
         r = psi_f64(v);
+    } else if (kind == 4) {
+        r = psi_sp1_f64(v - 1.0);
     } else {
         r = psi_appr_ref<double>(v);
     }

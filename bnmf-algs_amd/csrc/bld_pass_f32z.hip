@@ -12,8 +12,10 @@
 // stays in registers between the two phases instead of being read again.
 // It serves instances whose S stays nonnegative (S0 >= 0 and beta >= 1, the
 // descriptor's nonneg flag), so psi(S+1) always takes its x >= 1 form; other
-// instances run the generic pass. Results are bitwise identical to the
-// generic pass (same operations in the same order).
+// instances run the generic pass. The update is the generic pass's formula
+// with X_reciprocal and X / (S_ij+ + eps) folded into per-fiber constants
+// (nom and den scaled once per fiber, one reciprocal per element), so the
+// two kernels agree to FP32 rounding rather than bit for bit.
 //
 // Reference operations (src/bld/bld_mult/bld_mult_funcs.hpp): grad_plus
 // :177-192, nom :242-259, denom :272-290, update_S :307-327; the axis sums of

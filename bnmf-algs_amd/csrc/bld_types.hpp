@@ -21,13 +21,22 @@
 
 namespace bnmf {
 
-// Fibers (columns j) per tile: one wavefront lane per fiber.
+// Fibers (columns j) per tile: one wavefront lane per fiber (FP32 passes).
 constexpr uint32_t kTileCols = 64;
+// FP64 step pass: two lanes per fiber, so a tile row holds 32 fibers (four
+// lanes and 16 fibers for 32 < z <= 64).
+constexpr uint32_t kTileColsF64 = 32;
 // Rows per prep chunk (alpha_eph column-sum granularity) and columns per
 // prep chunk: small, so that even one large instance gives the prep launch
 // hundreds of blocks.
 constexpr uint32_t kPrepRows = 16;
 constexpr uint32_t kPrepCols = 16;
+// Deepest fiber the fused passes hold in registers; deeper instances
+// (z > kFusedZ) run the wide pass: one wavefront per fiber, lanes over k, on
+// tiles of kWideRows x kWideCols fibers.
+constexpr uint32_t kFusedZ = 64;
+constexpr uint32_t kWideRows = 16;
+constexpr uint32_t kWideCols = 16;
 
 struct BldInst {
     void* S;              // updated tensor (pass output)
@@ -54,6 +63,7 @@ struct BldInst {
     uint32_t hs, hm;      // lane l visits its fiber's chunks in the order
                           // m ^ ((l >> hs) & hm): distinct LDS banks per step
     uint32_t ti_rows;     // rows per pass tile
+    uint32_t tile_cols;   // fibers per tile row (kTileCols, kTileColsF64 or kWideCols)
     uint32_t n_ti, n_tj;  // tiles along rows / columns
     uint32_t nA;          // prep row chunks
     uint32_t n_cj;        // prep column chunks

@@ -57,6 +57,14 @@ hipError_t launch_init_s(int dtype, void* S, const void* X, size_t n_fibers, uin
 hipError_t init_kernel_attributes();
 hipError_t launch_fact(int dtype, const BldInst* inst, uint32_t y, uint32_t z, double eps, double* W,
                        double* H, double* L, hipStream_t st);
+// FP64 step pass (MODE_UPDATE or MODE_ADD), two lanes per fiber, z bucket
+// zb <= 64 and LDS read width rw (bld_pass_f64.hip)
+hipError_t launch_pass_f64(int psi, int mode, int zb, int rw, dim3 grid, hipStream_t st, const BldInst* insts,
+                           const PassItem* items, const uint32_t* n_items);
+hipError_t init_f64_attributes();
+// any mode and dtype for instances deeper than kFusedZ (bld_wide.hip)
+hipError_t launch_pass_wide(int dtype, int psi, int mode, dim3 grid, hipStream_t st, const BldInst* insts,
+                            const PassItem* items, const uint32_t* n_items);
 // FP32 step pass (MODE_UPDATE or MODE_ADD) specialised on the exact fiber
 // depth z (bld_pass_f32z.hip)
 hipError_t launch_pass_f32z(int z, int mode, dim3 grid, hipStream_t st, const BldInst* insts,
@@ -87,8 +95,7 @@ hipError_t launch_sample(double* out, const double* W, const double* H, const do
                          uint32_t z, uint64_t seed, int mode, size_t n, hipStream_t st);
 
 // ----------------------------------------------------------- tile planner
-// Largest z handled by the fused pass (fiber registers are sized by bucket).
-constexpr uint32_t kMaxZ = 64;
+// z bucket of the fused generic pass (z <= kFusedZ); 0 for the wide pass
 uint32_t z_bucket(uint32_t z);
 uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z);
 // Lays the items of one pass launch out XCD by XCD (L2 sharing of the
@@ -139,6 +146,7 @@ struct InstSpec {
 // Byte layout of one instance inside an arena.
 struct InstLayout {
     uint32_t x, y, z, zb, rw, hs, hm, ti_rows, n_ti, n_tj, nA, n_cj;
+    uint32_t tile_cols;  // fibers per tile row: kTileCols, or kWideCols (z > kFusedZ)
     size_t off_S, off_S0, off_X, off_alpha, off_beta, off_psib, off_psia, off_partA, off_psiA, off_pjk,
         off_ipk, off_score, off_result;
     size_t bytes;  // total, 256-byte aligned
@@ -150,16 +158,21 @@ InstLayout plan_instance(int dtype, uint32_t x, uint32_t y, uint32_t z, bool wit
 BldInst make_desc(const InstLayout& L, void* base, double eps, double a, double b, bool first,
                   bool nonneg = false);
 
-// Kernel of one pass launch: the exact-z FP32 update kernel (zexact > 0) or
-// the generic z-bucket kernel (zb, LDS read width rw). Instances with equal
-// keys share a launch. The key depends on the instance shape only, so an
-// instance runs the same kernel alone, batched or served.
+// Kernel of one pass launch: the exact-z FP32 step kernel (zexact > 0), the
+// FP64 step kernel (f64, z bucket zb, LDS read width rw), the generic
+// z-bucket kernel (zb, rw) or, for z > kFusedZ, the wide kernel (zb = 0).
+// Instances with equal keys share a launch. The key depends on the instance
+// shape only, so an instance runs the same kernel alone, batched or served.
 struct PassKey {
-    int zexact = 0, zb = 0, rw = 0;
+    int zexact = 0, zb = 0, rw = 0, f64 = 0;
     bool operator<(const PassKey& o) const {
-        return std::tie(zexact, zb, rw) < std::tie(o.zexact, o.zb, o.rw);
+        return std::tie(zexact, zb, rw, f64) < std::tie(o.zexact, o.zb, o.rw, o.f64);
     }
+    bool wide() const { return zexact == 0 && zb == 0; }
 };
+// fibers per tile row of an instance: kWideCols for z > kFusedZ, else
+// kTileColsF64 (FP64, z <= 32), half that (FP64, z <= 64) or kTileCols (FP32)
+uint32_t tile_cols_of(int dtype, uint32_t z);
 // nonneg: the instance's S stays >= 0 (S0 >= 0 and beta >= 1)
 PassKey pass_key(int dtype, int mode, const InstLayout& L, bool nonneg);
 hipError_t launch_pass_key(const PassKey& k, int dtype, int psi, int mode, dim3 grid,

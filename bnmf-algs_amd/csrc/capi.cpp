@@ -55,16 +55,15 @@ void check_dtype(int dtype) {
 }
 
 // check_bld_params (src/bld/util_details.hpp:20-28): X nonnegative; the
-// alpha/beta lengths are x and z by construction of this ABI.
+// alpha/beta lengths are x and z by construction of this ABI. Like the
+// reference, no bound on z: z = 0 is an empty instance (S has no elements).
 template <typename T>
 void check_params(const T* X, size_t x, size_t y, size_t z, const double* alpha,
                   const double* beta) {
     BNMF_REQUIRE(x * y == 0 || X != nullptr, "X is null");
     BNMF_REQUIRE(x == 0 || alpha != nullptr, "alpha is null");
     BNMF_REQUIRE(z == 0 || beta != nullptr, "beta is null");
-    BNMF_REQUIRE(z >= 1, "z must be at least 1");
-    BNMF_REQUIRE(z <= bnmf::kMaxZ, "z above 64 is not supported");
-    BNMF_REQUIRE(x < (1u << 31) && y < (1u << 31), "x and y must be below 2^31");
+    BNMF_REQUIRE(x < (1u << 31) && y < (1u << 31) && z < (1u << 31), "x, y and z must be below 2^31");
     for (size_t t = 0; t < x * y; ++t)
         BNMF_REQUIRE(X[t] >= T(0), "X must be nonnegative");
 }
@@ -101,6 +100,9 @@ void appr_one(int dtype, const T* X, size_t x, size_t y, size_t z, const double*
                      const double* beta, size_t max_iter, double eps, const T* nu0, const T* mu0,
                      uint64_t seed, T* S_out, T* nu_out, T* mu_out) {
     check_params(X, x, y, z, alpha, beta);
+    // z = 0: nu, mu and S have no elements
+    if (z == 0) return;
+    BNMF_REQUIRE(z <= bnmf::kFusedZ, "bld_appr supports z in [1, 64]");
     // an empty X (x*y = 0) runs no iteration; the factors come back as
     // initialised, as they do for an empty instance of a batch
     bnmf::ApprSpec s;
@@ -255,7 +257,8 @@ bnmf_status bnmf_log_marginal_S(int dtype, const void* S, size_t x, size_t y, si
         BNMF_REQUIRE(out != nullptr, "out is null");
         BNMF_REQUIRE(x * y * z == 0 || S != nullptr, "S is null");
         BNMF_REQUIRE(x == 0 || alpha != nullptr, "alpha is null");
-        BNMF_REQUIRE(beta != nullptr && z >= 1 && z <= bnmf::kMaxZ, "beta is null or z out of range");
+        BNMF_REQUIRE(z == 0 || beta != nullptr, "beta is null");
+        BNMF_REQUIRE(x < (1u << 31) && y < (1u << 31) && z < (1u << 31), "x, y and z must be below 2^31");
         // X is not used by the scoring pass; a zero matrix satisfies the
         // instance layout.
         std::vector<char> Xz(x * y * esize(dtype), 0);
@@ -279,7 +282,7 @@ bnmf_status bnmf_tensor_sums(int dtype, const void* S, size_t x, size_t y, size_
     return guarded([&] {
         check_dtype(dtype);
         BNMF_REQUIRE(x * y * z == 0 || S != nullptr, "S is null");
-        BNMF_REQUIRE(z >= 1 && z <= bnmf::kMaxZ, "z must be in [1, 64]");
+        BNMF_REQUIRE(x < (1u << 31) && y < (1u << 31) && z < (1u << 31), "x, y and z must be below 2^31");
         if (x * y * z == 0) return;
         std::vector<char> Xz(x * y * esize(dtype), 0);
         std::vector<double> ones_x(x, 1.0), ones_z(z, 1.0);
@@ -394,8 +397,8 @@ bnmf_status bnmf_bld_fact(int dtype, const void* S, size_t x, size_t y, size_t z
         check_dtype(dtype);
         BNMF_REQUIRE(x * y * z == 0 || S != nullptr, "S is null");
         BNMF_REQUIRE(x == 0 || alpha != nullptr, "alpha is null");
-        BNMF_REQUIRE(beta != nullptr && z >= 1 && z <= bnmf::kMaxZ, "beta is null or z out of range");
-        BNMF_REQUIRE(x < (1u << 31) && y < (1u << 31), "x and y must be below 2^31");
+        BNMF_REQUIRE(z == 0 || beta != nullptr, "beta is null");
+        BNMF_REQUIRE(x < (1u << 31) && y < (1u << 31) && z < (1u << 31), "x, y and z must be below 2^31");
         if (x * y * z == 0) return;
         // X is not read by the sums pass; a zero matrix satisfies the layout
         std::vector<char> Xz(x * y * esize(dtype), 0);
@@ -494,7 +497,7 @@ bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_
 
 bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n) {
     return guarded([&] {
-        BNMF_REQUIRE(kind >= 0 && kind <= 3, "kind must be 0..3");
+        BNMF_REQUIRE(kind >= 0 && kind <= 4, "kind must be 0..4");
         BNMF_REQUIRE(n == 0 || (x && out), "null argument");
         if (n == 0) return;
         bnmf::ensure_runtime();
@@ -509,7 +512,7 @@ bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n) {
 bnmf_status bnmf_tune_set(int dtype, size_t z, size_t fibers, unsigned rows) {
     return guarded([&] {
         check_dtype(dtype);
-        BNMF_REQUIRE(z >= 1 && z <= bnmf::kMaxZ, "z out of range");
+        BNMF_REQUIRE(z >= 1 && z <= bnmf::kFusedZ, "the tile table covers z in [1, 64] (deeper fibers use fixed tiles)");
         BNMF_REQUIRE(rows == 0 || (rows >= 4 && rows <= 1024), "rows must be 0 or in [4, 1024]");
         bnmf::tune_set(dtype, (uint32_t)z, (uint64_t)fibers, rows);
     });
@@ -519,7 +522,7 @@ bnmf_status bnmf_tune_get(int dtype, size_t z, size_t fibers, unsigned* rows) {
     return guarded([&] {
         check_dtype(dtype);
         BNMF_REQUIRE(rows != nullptr, "rows is null");
-        BNMF_REQUIRE(z >= 1 && z <= bnmf::kMaxZ, "z out of range");
+        BNMF_REQUIRE(z >= 1 && z <= bnmf::kFusedZ, "the tile table covers z in [1, 64] (deeper fibers use fixed tiles)");
         *rows = bnmf::tune_ti_cap(dtype, (uint32_t)z, (uint64_t)fibers);
     });
 }
@@ -539,7 +542,7 @@ bnmf_status bnmf_tune_save(const char* path) {
 }
 
 unsigned bnmf_tile_rows(int dtype, size_t x, size_t y, size_t z) {
-    if ((dtype != BNMF_F32 && dtype != BNMF_F64) || z < 1 || z > bnmf::kMaxZ || x == 0 || y == 0 ||
+    if ((dtype != BNMF_F32 && dtype != BNMF_F64) || z < 1 || z >= (1u << 31) || x == 0 || y == 0 ||
         x >= (1u << 31) || y >= (1u << 31))
         return 0;
     try {

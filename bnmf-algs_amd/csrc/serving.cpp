@@ -640,10 +640,11 @@ bnmf_status bnmf_server_submit(bnmf_server* s, const bnmf_bld_desc* req, size_t 
                                int want_log_marginal, uint64_t* ticket) {
     return guarded_s([&] {
         BNMF_REQUIRE(s && req && ticket, "null argument");
-        BNMF_REQUIRE(req->z >= 1 && req->z <= bnmf::kMaxZ, "z must be in [1, 64]");
         BNMF_REQUIRE(req->x * req->y == 0 || req->X != nullptr, "X is null");
-        BNMF_REQUIRE(req->alpha != nullptr && req->beta != nullptr, "alpha or beta is null");
-        BNMF_REQUIRE(req->x < (1u << 31) && req->y < (1u << 31), "x and y must be below 2^31");
+        BNMF_REQUIRE((req->x == 0 || req->alpha != nullptr) && (req->z == 0 || req->beta != nullptr),
+                     "alpha or beta is null");
+        BNMF_REQUIRE(req->x < (1u << 31) && req->y < (1u << 31) && req->z < (1u << 31),
+                     "x, y and z must be below 2^31");
         *ticket = s->impl->submit(*req, max_iter, want_log_marginal);
     });
 }
@@ -681,7 +682,7 @@ bnmf_status bnmf_server_get_stats(bnmf_server* s, bnmf_server_stats* out) {
 }
 
 size_t bnmf_server_request_bytes(int dtype, size_t x, size_t y, size_t z) {
-    if (z < 1 || z > bnmf::kMaxZ) return 0;
+    if (x >= (1u << 31) || y >= (1u << 31) || z >= (1u << 31)) return 0;
     return bnmf::Server::request_bytes(dtype, (uint32_t)x, (uint32_t)y, (uint32_t)z);
 }
 

@@ -53,7 +53,9 @@ bnmf_status bnmf_init(int device);
  * Exactly max_iter iterations run (the reference has no convergence test).
  * Parameter errors (X < 0 or NaN, null pointers) return
  * BNMF_ERR_INVALID_ARGUMENT, as check_bld_params asserts
- * (src/bld/util_details.hpp:20-28).
+ * (src/bld/util_details.hpp:20-28). As there, z has no upper bound (fibers
+ * deeper than 64 run a wavefront-per-fiber pass); z = 0, like x = 0 or
+ * y = 0, is an empty instance. x, y and z must each be below 2^31.
  * ---------------------------------------------------------------------- */
 /* Plain FP64 form: every step in double, exact digamma unless use_psi_appr. */
 bnmf_status bnmf_bld_mult_f64(const double* X, size_t x, size_t y, size_t z,
@@ -226,9 +228,10 @@ bnmf_status bnmf_tune_set(int dtype, size_t z, size_t fibers, unsigned rows);
 bnmf_status bnmf_tune_get(int dtype, size_t z, size_t fibers, unsigned* rows);
 bnmf_status bnmf_tune_load(const char* path);
 bnmf_status bnmf_tune_save(const char* path);
-/* Rows per tile an instance of this shape gets now (table or rule); 0 for a
- * shape with no tiles (x or y = 0) or outside the solver's range (an unknown
- * dtype, z outside [1, 64], x or y at or above 2^31). */
+/* Rows per tile an instance of this shape gets now (table or rule; 16 for
+ * z > 64, whose wide pass has fixed 16 x 16 tiles); 0 for a shape with no
+ * tiles (x, y or z = 0) or outside the solver's range (an unknown dtype, x,
+ * y or z at or above 2^31). The table covers z in [1, 64]. */
 unsigned bnmf_tile_rows(int dtype, size_t x, size_t y, size_t z);
 /* Path of the loaded table file, "" if none (valid until the next load). */
 const char* bnmf_tune_source(void);
@@ -272,7 +275,8 @@ bnmf_status bnmf_server_wait(bnmf_server* s, uint64_t ticket, double timeout_s);
 bnmf_status bnmf_server_collect(bnmf_server* s, uint64_t ticket, void* S_out,
                                 double* log_marginal_out, bnmf_request_stats* stats);
 bnmf_status bnmf_server_get_stats(bnmf_server* s, bnmf_server_stats* out);
-/* HBM bytes a request of this shape holds while admitted (0 if z > 64). */
+/* HBM bytes a request of this shape holds while admitted (0 if x, y or z is
+ * 2^31 or more). */
 size_t bnmf_server_request_bytes(int dtype, size_t x, size_t y, size_t z);
 void bnmf_server_destroy(bnmf_server* s);
 
@@ -304,7 +308,8 @@ bnmf_status bnmf_init_S(int dtype, uint64_t seed, const void* X, size_t x, size_
 
 /* Evaluates the device digamma forms on the GPU (kind 0: FP32 bulk psi,
  * 1: exact FP64, 2: util::psi_appr in FP64, src/util/util.hpp:440-514,
- * 3: FP32 psi(s+1) of the update passes evaluated at x = s+1).
+ * 3: FP32 psi(s+1) of the update passes evaluated at x = s+1, 4: FP64
+ * psi(s+1) of the update passes evaluated at x = s+1).
  * x and out are host arrays of n doubles. */
 bnmf_status bnmf_psi_eval(int kind, const double* x, double* out, size_t n);
 

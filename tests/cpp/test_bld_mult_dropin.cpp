@@ -44,8 +44,10 @@ static bool no_throw(const std::function<void()>& f) {
     return true;
 }
 
+// deep: the solver accepts z > 64 as the reference does (bld_appr's passes
+// hold at most 64 components and reject deeper instances)
 template <typename Solve>
-static void parameter_checks(Solve solve, bool run_solves) {
+static void parameter_checks(Solve solve, bool run_solves, bool deep = true) {
     size_t x = 10, y = 5, z = 2;
     shape<3> tensor_shape{x, y, z};
     matrixd X = matrixd::Constant(x, y, 5);
@@ -61,13 +63,21 @@ static void parameter_checks(Solve solve, bool run_solves) {
     z--;
     X = matrixd::Constant(x + 1, y, 5);
     CHECK(throws([&] { solve(X, z, model_params, 1000); }));
-    // beyond the reference: NaN input and z outside the supported range
+    // beyond the reference: NaN input
     X = matrixd::Constant(x, y, 5);
     X(3, 2) = std::nan("");
     CHECK(throws([&] { solve(X, z, model_params, 5); }));
     X(3, 2) = 1;
+    // no bound on z (check_bld_params, src/bld/util_details.hpp:20-28)
     alloc_model::Params<double> wide(shape<3>{x, y, 65});
-    CHECK(throws([&] { solve(X, 65, wide, 5); }));
+    if (!deep) {
+        CHECK(throws([&] { solve(X, 65, wide, 5); }));
+    } else if (run_solves) {
+        CHECK(no_throw([&] {
+            const auto S = solve(X, 65, wide, 5);
+            if (S.dimension(2) != 65) throw std::runtime_error("wrong depth");
+        }));
+    }
 }
 
 static double sum_gap(const matrixd& X, const tensord<3>& S) {
@@ -272,7 +282,7 @@ int main(int argc, char** argv) {
     auto appr = [](const matrixd& X, size_t z, const alloc_model::Params<double>& p, size_t it) {
         return std::get<0>(bld::bld_appr(X, z, p, it));
     };
-    parameter_checks(appr, gpu);
+    parameter_checks(appr, gpu, false);
     if (gpu) {
         auto mult0 = [](const matrixd& X, size_t z, const alloc_model::Params<double>& p,
                         auto&&... rest) { return bld::bld_mult(X, z, p, rest...); };

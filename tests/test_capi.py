@@ -64,12 +64,15 @@ def test_c_abi_reports_invalid_argument(hip):
                                a.ctypes.data_as(hip._D), 5, 0, 1e-50, None, 0, out.ctypes.data)
     assert st == hip.ERR_INVALID_ARGUMENT
     assert b"nonnegative" in lib.bnmf_last_error()
-    st = lib.bnmf_bld_mult_f64(X.ctypes.data, 1, 2, 0, a.ctypes.data_as(hip._D),
+    # like check_bld_params, no upper bound on z (z > 64 runs the wide pass,
+    # tested on the GPU); z = 0 is an empty instance with nothing to compute
+    good = np.array([[1.0, 2.0]])
+    st = lib.bnmf_bld_mult_f64(good.ctypes.data, 1, 2, 0, a.ctypes.data_as(hip._D), None, 5, 0,
+                               1e-50, None, 0, None)
+    assert st == hip.OK
+    st = lib.bnmf_bld_mult_f64(good.ctypes.data, 1, 2, 1 << 31, a.ctypes.data_as(hip._D),
                                a.ctypes.data_as(hip._D), 5, 0, 1e-50, None, 0, out.ctypes.data)
-    assert st == hip.ERR_INVALID_ARGUMENT
-    st = lib.bnmf_bld_mult_f64(X.ctypes.data, 1, 2, 65, a.ctypes.data_as(hip._D),
-                               a.ctypes.data_as(hip._D), 5, 0, 1e-50, None, 0, out.ctypes.data)
-    assert st == hip.ERR_INVALID_ARGUMENT
+    assert st == hip.ERR_INVALID_ARGUMENT and b"below 2^31" in lib.bnmf_last_error()
 
 
 def test_sibling_solvers_report_invalid_argument(hip):
@@ -90,8 +93,12 @@ def test_sibling_solvers_report_invalid_argument(hip):
     for name, call in calls.items():
         assert call(neg, 1) == hip.ERR_INVALID_ARGUMENT, name
         assert b"nonnegative" in lib.bnmf_last_error(), name
-        assert call(good, 0) == hip.ERR_INVALID_ARGUMENT, name
-        assert call(good, 65) == hip.ERR_INVALID_ARGUMENT, name
+        # z = 0: an empty instance
+        assert call(good, 0) == hip.OK, name
+        assert call(good, 1 << 31) == hip.ERR_INVALID_ARGUMENT, name
+    # bld_appr's passes hold a tile's factors for z <= 64
+    assert calls["appr"](good, 65) == hip.ERR_INVALID_ARGUMENT
+    assert b"z in [1, 64]" in lib.bnmf_last_error()
     # null X with a nonempty shape
     st = lib.bnmf_bld_add_f64(None, 1, 2, 1, A, A, 5, 1e-50, None, 0, O)
     assert st == hip.ERR_INVALID_ARGUMENT and b"X is null" in lib.bnmf_last_error()

@@ -44,6 +44,24 @@ def test_device_psi_exact(hip, golden):
     assert np.allclose(neg, golden["psi_neg"]["psi"], rtol=1e-12, atol=1e-12)
 
 
+def test_device_psi_sp1_f64(hip, golden):
+    # the FP64 update's psi(S+1): a rational fit on [1, 2) reached by
+    # recurrence from x < 10, psi_f64 elsewhere; against mpmath on a dense
+    # grid (every shift count, both ends of each unit interval) and the
+    # golden vectors at x >= 1
+    import mpmath
+    xs = np.concatenate([np.linspace(1.0, 10.0, 1801)[:-1], 1.0 + np.arange(1, 10) * 1e-12,
+                         np.arange(2.0, 10.0) - 1e-13, [1.4616321449683622, 9.999999999, 10.0, 37.5, 1e6]])
+    ref = np.array([float(mpmath.digamma(mpmath.mpf(float(v)))) for v in xs])
+    got = hip.psi_eval(4, xs)
+    assert np.all(np.abs(got - ref) <= 8e-16 * np.maximum(1.0, np.abs(ref)))
+    gx = np.array(golden["psi"]["x"])
+    gp = np.array(golden["psi"]["psi"])
+    keep = gx >= 1.0
+    got = hip.psi_eval(4, gx[keep])
+    assert np.all(np.abs(got - gp[keep]) <= 3e-14 * np.maximum(1, np.abs(gp[keep])))
+
+
 def test_device_psi_appr(hip, golden):
     xs = np.array(golden["psi_appr"]["x"])
     ref = np.array(golden["psi_appr"]["psi_appr"])

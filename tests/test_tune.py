@@ -43,7 +43,9 @@ def test_invalid_entries(table):
         hip.tune_set(np.float32, 8, 100, 3)
     with pytest.raises(ValueError):
         hip.tune_set(np.float32, 65, 100, 64)
-    assert hip.tile_rows(np.float32, 10, 10, 65) == 0
+    # deeper fibers run the wide pass on fixed 16 x 16 tiles
+    assert hip.tile_rows(np.float32, 10, 10, 65) == 16
+    assert hip.tile_rows(np.float32, 10, 10, 0) == 0
 
 
 def test_save_load_roundtrip(table, tmp_path):
