@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                                 const uint32_t mm = (m0 + (u ^ xm)) * RW;
                                 for (int v = 0; v < RW; ++v) {
                                     const double xa = fb[mm + v] + 1.0;
-                                    if (!(xa >= 1.0 && xa < 10.0)) gb[mm + v] = pbb[mm + v] - psi_f64(xa);
+                                    if (!(xa >= 1.0 && xa < 10.0)) gb[mm + v] = pbb[mm + v] - psi_f64_general(xa);
                                 }
                             }
                         }

@@ -4,14 +4,14 @@
 //  * psi_f64        : the exact digamma in double, the role GSL's gsl_sf_psi
 //                     plays for the reference CPU solver
 //                     (src/bld/util_details.hpp:71-73, bld_mult.hpp:119-120).
-//                     Recurrence psi(x) = psi(x+1) - 1/x (Abramowitz & Stegun
-//                     6.3.5) lifts x to >= 10; the shift sum is accumulated as
-//                     one fraction so that a single division is paid, then the
+//                     On [1, 10) (almost every argument of a solve) a
+//                     rational fit on [1, 2) reached by downward recurrence
+//                     (one division, no log); elsewhere the recurrence
+//                     psi(x) = psi(x+1) - 1/x (Abramowitz & Stegun 6.3.5)
+//                     lifts x to >= 10, the shift sum accumulated as one
+//                     fraction so that a single division is paid, then the
 //                     asymptotic series ln x - 1/(2x) - sum B_2n/(2n x^2n)
 //                     (A&S 6.3.18) is summed to the x^-16 term.
-//  * psi_sp1_f64    : psi(S + 1) for the FP64 update pass: a rational fit on
-//                     [1, 2) reached by downward recurrence from x < 10 (one
-//                     division, no log), psi_f64 elsewhere.
 //  * psi_appr_ref   : util::psi_appr (src/util/util.hpp:440-514), the
 //                     reference's approximate digamma: the same case split
 //                     and the same closed-form series, evaluated in T. The
@@ -35,7 +35,35 @@
 namespace bnmf {
 
 // ---------------------------------------------------------------- FP64 exact
-__host__ __device__ inline double psi_f64(double x) {
+// psi(1 + t) for t in [0, 1): P(t) / Q(t) with P, Q of degree 6, fitted for
+// this project to mpmath's digamma (Loeb's linearised least squares with
+// Lawson reweighting towards the minimax error; max |error| 2.3e-19 in exact
+// arithmetic, about one ulp of the result when evaluated in double).
+__host__ __device__ inline double psi1p_rat(double t, double* q_out = nullptr) {
+    double p = 2.2740920435799464e-04;
+    p = fma(p, t, 1.2358749079795044e-02);
+    p = fma(p, t, 1.6310050654079108e-01);
+    p = fma(p, t, 7.8211906069224120e-01);
+    p = fma(p, t, 1.3871517644465865e+00);
+    p = fma(p, t, 4.2674224942732580e-01);
+    p = fma(p, t, -5.7721566490153290e-01);
+    double q = 4.0209462601665044e-05;
+    q = fma(q, t, 3.4653371266403213e-03);
+    q = fma(q, t, 6.6582844787605800e-02);
+    q = fma(q, t, 4.8134849233954260e-01);
+    q = fma(q, t, 1.5286529550625810e+00);
+    q = fma(q, t, 2.1104621573787530e+00);
+    q = fma(q, t, 1.0);
+    if (q_out) {
+        *q_out = q;
+        return p;
+    }
+    return p / q;
+}
+
+// Any argument: the recurrence to x >= 10 and the asymptotic series, the
+// reflection formula for x <= 0.
+__host__ __device__ inline double psi_f64_general(double x) {
     if (!(x > 0.0)) {
         if (x != x) return x;
         if (x == floor(x)) return __builtin_nan("");
@@ -75,39 +103,6 @@ __host__ __device__ inline double psi_f64(double x) {
     return (log(x) - 0.5 * r - series) - num / den;
 }
 
-// psi(1 + t) for t in [0, 1): P(t) / Q(t) with P, Q of degree 6, fitted for
-// this project to mpmath's digamma (Loeb's linearised least squares with
-// Lawson reweighting towards the minimax error; max |error| 2.3e-19 in exact
-// arithmetic, about one ulp of the result when evaluated in double).
-__host__ __device__ inline double psi1p_rat(double t, double* q_out = nullptr) {
-    double p = 2.2740920435799464e-04;
-    p = fma(p, t, 1.2358749079795044e-02);
-    p = fma(p, t, 1.6310050654079108e-01);
-    p = fma(p, t, 7.8211906069224120e-01);
-    p = fma(p, t, 1.3871517644465865e+00);
-    p = fma(p, t, 4.2674224942732580e-01);
-    p = fma(p, t, -5.7721566490153290e-01);
-    double q = 4.0209462601665044e-05;
-    q = fma(q, t, 3.4653371266403213e-03);
-    q = fma(q, t, 6.6582844787605800e-02);
-    q = fma(q, t, 4.8134849233954260e-01);
-    q = fma(q, t, 1.5286529550625810e+00);
-    q = fma(q, t, 2.1104621573787530e+00);
-    q = fma(q, t, 1.0);
-    if (q_out) {
-        *q_out = q;
-        return p;
-    }
-    return p / q;
-}
-
-// psi(s + 1), the per-element term of the FP64 update, at the rounded
-// argument x = s + 1 as the reference evaluates it. For 1 <= x < 10
-// (almost every element of a solve: S_ijk <= X_ij) the recurrence
-// psi(x) = psi(x - n) + sum_{m=1..n} 1/(x - m) (Abramowitz & Stegun 6.3.5)
-// brings x down to [1, 2), where psi1p_rat applies; the shift sum is kept as
-// one fraction num/den and joined to P/Q, so one division is paid. Other
-// arguments take psi_f64.
 // psi(x) for 1 <= x < 10 (the caller guarantees the range): the recurrence
 // psi(x) = psi(x - n) + sum_{m=1..n} 1/(x - m) (Abramowitz & Stegun 6.3.5)
 // brings x down to [1, 2), where psi1p_rat applies; the shift sum is kept as
@@ -126,14 +121,18 @@ __host__ __device__ inline double psi_1to10(double x) {
     return fma(p, den, q * num) / (q * den);
 }
 
-// psi(s + 1), the per-element term of the FP64 update, at the rounded
-// argument x = s + 1 as the reference evaluates it: psi_1to10 for
-// 1 <= x < 10 (almost every element of a solve, S_ijk <= X_ij), psi_f64
-// elsewhere.
-__host__ __device__ inline double psi_sp1_f64(double s) {
-    const double x = s + 1.0;
-    return (x >= 1.0 && x < 10.0) ? psi_1to10(x) : psi_f64(x);
+// The exact digamma of the FP64 forms: psi_1to10 on [1, 10), psi_f64_general
+// elsewhere. Every FP64 digamma (prep's psi(beta_eph), psi(alpha_eph) and
+// psi(A), the pass's psi(S + 1)) is this one function, so equal arguments
+// give equal values, as in the reference (a single row with beta = 1 has
+// psi(beta_eph) = psi(S + 1) exactly, hence a zero gradient).
+__host__ __device__ inline double psi_f64(double x) {
+    return (x >= 1.0 && x < 10.0) ? psi_1to10(x) : psi_f64_general(x);
 }
+
+// psi(s + 1), the per-element term of the FP64 update, at the rounded
+// argument x = s + 1 as the reference evaluates it.
+__host__ __device__ inline double psi_sp1_f64(double s) { return psi_f64(s + 1.0); }
 
 // ------------------------------------------------- reference psi_appr form
 template <typename Real>

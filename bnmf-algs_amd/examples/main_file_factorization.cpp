@@ -1,18 +1,30 @@
 // main_file_factorization.cpp — command-line factorization of a matrix file
-// with the GPU bld_mult solver (the bld_mult branch of the reference's
-// example/main_file_factorization.cpp:19-119).
+// with the library's GPU solvers (the bld_mult, bld_add and bld_appr branches
+// of the reference's example/main_file_factorization.cpp:19-119).
 //
-//   main_file_factorization bld_mult FILE N_COMPONENTS BETA MAX_ITER
+//   main_file_factorization ALG FILE N_COMPONENTS BETA MAX_ITER
 //
-// FILE holds one matrix row per line, entries separated by whitespace. The
-// model parameters are the example's: a = 100, b = 1, alpha = 0.05 (x
-// entries), beta = 10 (z entries, the last 60); bld_mult runs with
-// psi_appr, then bld_fact. Writes S.txt (x*y lines of z values: fiber S_ij:
-// per line, rows i outer), W.txt (x lines of z), H.txt (z lines of y) and
-// L.txt (one line of y) to the working directory. BETA (the NMF beta of the
-// reference's other algorithms) is accepted and unused by bld_mult.
+// ALG is bld_mult, bld_add or bld_appr (the reference's other algorithms,
+// nmf, seq_greedy_bld, collapsed_gibbs and collapsed_icm, are not part of
+// this library). FILE holds one matrix row per line, entries separated by
+// whitespace. The model parameters are the example's: a = 100, b = 1,
+// alpha = 0.05 (x entries), beta = 10 (z entries, the last 60); bld_mult runs
+// with psi_appr; each solver is followed by bld_fact. BETA (the NMF beta of
+// the reference's nmf branch) is accepted and unused.
+//
+// S.txt, W.txt, H.txt and L.txt are written to the working directory in the
+// layout the reference's `file << tensor` / `file << matrix` produces
+// (Eigen's default stream format): one line per index of the first
+// dimension (S: x lines of y*z values, j outer and k inner; W: x lines of z;
+// H: z lines of y; L: one line of y), values in the stream's default
+// notation and precision (6 significant digits), each right-aligned to the
+// widest value of the object and separated by one space, lines joined by
+// newlines with none after the last. BNMF_CLI_PRECISION=<digits> (e.g. 17)
+// writes more digits in the same layout.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -48,13 +60,26 @@ bool read_matrix(const std::string& path, std::vector<double>& data, long& rows,
     return rows > 0;
 }
 
-template <typename M>
-void write_rows(const std::string& path, const M& m, long rows, long cols) {
+// Eigen's default stream output of a rows x cols row-major block: every
+// value formatted as the stream would, padded on the left to the widest one
+void write_eigen(const std::string& path, const double* m, long rows, long cols, int precision) {
     std::ofstream out(path, std::ios_base::trunc);
-    out << std::setprecision(17);
+    if (rows * cols == 0) return;
+    std::vector<std::string> txt((size_t)(rows * cols));
+    size_t width = 0;
+    for (long t = 0; t < rows * cols; ++t) {
+        std::ostringstream v;
+        v.precision(precision);
+        v << m[t];
+        txt[(size_t)t] = v.str();
+        width = std::max(width, txt[(size_t)t].size());
+    }
     for (long r = 0; r < rows; ++r) {
-        for (long c = 0; c < cols; ++c) out << (c ? " " : "") << m.data()[r * cols + c];
-        out << '\n';
+        for (long c = 0; c < cols; ++c) {
+            if (c) out << ' ';
+            out << std::setw((int)width) << txt[(size_t)(r * cols + c)];
+        }
+        if (r + 1 < rows) out << '\n';
     }
 }
 
@@ -63,12 +88,13 @@ void write_rows(const std::string& path, const M& m, long rows, long cols) {
 int main(int argc, char** argv) {
     if (argc != 6) {
         std::cout << "usage: " << argv[0] << " alg filename n_components beta max_iter\n\n"
-                  << "where alg is bld_mult (the GPU solver of this library)" << std::endl;
+                  << "where alg is one of <bld_mult | bld_add | bld_appr>" << std::endl;
         return -1;
     }
     const std::string alg(argv[1]);
-    if (alg != "bld_mult") {
-        std::cout << "Invalid algorithm: this build provides bld_mult only" << std::endl;
+    if (alg != "bld_mult" && alg != "bld_add" && alg != "bld_appr") {
+        std::cout << "Invalid algorithm: this library provides bld_mult, bld_add and bld_appr"
+                  << std::endl;
         return -1;
     }
     const std::string filename(argv[2]);
@@ -96,7 +122,12 @@ int main(int argc, char** argv) {
     matrixd W, H;
     vectord L;
     try {
-        S = bld::bld_mult(X, z, params, max_iter, true);
+        if (alg == "bld_mult")
+            S = bld::bld_mult(X, z, params, max_iter, true);
+        else if (alg == "bld_add")
+            S = bld::bld_add(X, z, params, max_iter);
+        else
+            S = std::get<0>(bld::bld_appr(X, z, params, max_iter));
         std::tie(W, H, L) = bld::bld_fact(S, params);
     } catch (const std::exception& e) {
         std::cout << "error: " << e.what() << std::endl;
@@ -107,9 +138,11 @@ int main(int argc, char** argv) {
               << std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count()
               << " milliseconds" << std::endl;
 
-    write_rows("S.txt", S, x * y, (long)z);
-    write_rows("W.txt", W, x, (long)z);
-    write_rows("H.txt", H, (long)z, y);
-    write_rows("L.txt", L, 1, y);
+    int precision = 6;
+    if (const char* e = std::getenv("BNMF_CLI_PRECISION")) precision = std::max(1, std::atoi(e));
+    write_eigen("S.txt", S.data(), x, y * (long)z, precision);
+    write_eigen("W.txt", W.data(), x, (long)z, precision);
+    write_eigen("H.txt", H.data(), (long)z, y, precision);
+    write_eigen("L.txt", L.data(), 1, y, precision);
     return 0;
 }

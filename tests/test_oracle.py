@@ -217,6 +217,44 @@ def test_sparseness_reference_cases(oracle):
 
 
 # ------------------------------------------------------------------ bld_fact
+def test_bld_fact_is_the_posterior_mode(oracle):
+    """bld_fact (bld_fact.hpp:36-80) returns the modes of the allocation
+    model's conditional posteriors: W[:, k] ~ Dirichlet(alpha + S_i+k),
+    H[:, j] ~ Dirichlet(beta + S_+jk), L_j ~ Gamma(a + S_+j+, rate b + 1).
+    Each mode is found here by numerical maximisation of the log density
+    (softmax parameterisation of the simplex, scipy.optimize), independently
+    of the closed form, and compared with the oracle's factors; columns of W
+    and H sum to 1."""
+    from scipy.optimize import minimize, minimize_scalar
+    from scipy.stats import gamma
+    rng = np.random.default_rng(11)
+    x, y, z = 5, 4, 3
+    S = rng.uniform(0, 3, (x, y, z))
+    alpha = rng.uniform(1.2, 2.5, x)
+    beta = rng.uniform(1.2, 3.0, z)
+    a, b = 4.0, 0.7
+    W, H, L = oracle.bld_fact(S, alpha, beta, a, b)
+
+    def dirichlet_mode(conc):
+        # maximise sum (c_i - 1) log w_i over the simplex, w = softmax(u)
+        def neg(u):
+            lw = u - np.logaddexp.reduce(u)
+            return -np.sum((conc - 1.0) * lw)
+        res = minimize(neg, np.zeros(len(conc)), method="BFGS", options={"gtol": 1e-12})
+        return np.exp(res.x - np.logaddexp.reduce(res.x))
+
+    for k in range(z):
+        assert np.allclose(W[:, k], dirichlet_mode(alpha + S[:, :, k].sum(1)), atol=1e-7), k
+    for j in range(y):
+        assert np.allclose(H[:, j], dirichlet_mode(beta + S[:, j, :].sum(0)), atol=1e-7), j
+    for j in range(y):
+        shape = a + S[:, j, :].sum()
+        r = minimize_scalar(lambda v: -gamma.logpdf(v, shape, scale=1.0 / (b + 1.0)),
+                            bounds=(1e-9, 10 * shape), method="bounded", options={"xatol": 1e-12})
+        assert abs(L[j] - r.x) <= 1e-6 * r.x, j
+    assert np.allclose(W.sum(0), 1.0, rtol=1e-14) and np.allclose(H.sum(0), 1.0, rtol=1e-14)
+
+
 def test_bld_fact_closed_form(oracle):
     rng = np.random.default_rng(4)
     S = rng.uniform(0, 3, (4, 5, 3))
