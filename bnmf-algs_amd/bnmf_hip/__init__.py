@@ -90,6 +90,10 @@ def lib():
         L.bnmf_bld_mult_f64.restype = st
         L.bnmf_bld_mult_f32.argtypes = L.bnmf_bld_mult_f64.argtypes
         L.bnmf_bld_mult_f32.restype = st
+        # device pointers (X, alpha, beta, S0, S_out) and a hipStream_t
+        L.bnmf_bld_mult_f32_noalloc.argtypes = [_VP, _SZ, _SZ, _SZ, _VP, _VP, _SZ, ctypes.c_int,
+                                                ctypes.c_double, _VP, ctypes.c_uint64, _VP, _VP]
+        L.bnmf_bld_mult_f32_noalloc.restype = st
         L.bnmf_bld_add_f64.argtypes = [_VP, _SZ, _SZ, _SZ, _D, _D, _SZ, ctypes.c_double, _VP,
                                        ctypes.c_uint64, _VP]
         L.bnmf_bld_add_f64.restype = st

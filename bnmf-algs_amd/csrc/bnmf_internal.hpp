@@ -102,6 +102,13 @@ uint32_t choose_ti_rows(int dtype, uint32_t x, uint32_t y, uint32_t z);
 // per-column psi staging); results do not depend on item order.
 void xcd_order(std::vector<PassItem>& items);
 
+// ------------------------------------------------------ no-allocation solve
+// bld_mult's FP32 solve on caller-owned device buffers, workspace in the
+// library's fixed device scratch (noalloc.hip); enqueued on st
+void bld_mult_f32_noalloc(const float* X, uint32_t x, uint32_t y, uint32_t z, const double* alpha,
+                          const double* beta, size_t max_iter, double eps, const float* S0, uint64_t seed,
+                          float* S, hipStream_t st);
+
 // ------------------------------------------------------------- tile table
 // Rows per pass tile by shape class (tune.cpp); 0 = built-in rule.
 int tune_zclass(uint32_t z);

@@ -157,6 +157,22 @@ bnmf_status bnmf_bld_mult_f32(const float* X, size_t x, size_t y, size_t z, cons
     });
 }
 
+bnmf_status bnmf_bld_mult_f32_noalloc(const float* X, size_t x, size_t y, size_t z, const double* alpha,
+                                      const double* beta, size_t max_iter, int use_psi_appr, double eps,
+                                      const float* S0, uint64_t seed, float* S_out, void* stream) {
+    (void)use_psi_appr;  // the FP32 form has one digamma, as in bnmf_bld_mult_f32
+    return guarded([&] {
+        BNMF_REQUIRE(x < (1u << 31) && y < (1u << 31) && z < (1u << 31), "x, y and z must be below 2^31");
+        if (x * y * z == 0) return;
+        BNMF_REQUIRE(X && alpha && beta && S_out, "null argument");
+        BNMF_REQUIRE((((uintptr_t)X | (uintptr_t)S0 | (uintptr_t)S_out) & 15u) == 0,
+                     "X, S0 and S_out must be 16-byte aligned device buffers");
+        BNMF_REQUIRE((((uintptr_t)alpha | (uintptr_t)beta) & 7u) == 0, "alpha and beta must be 8-byte aligned");
+        bnmf::bld_mult_f32_noalloc(X, (uint32_t)x, (uint32_t)y, (uint32_t)z, alpha, beta, max_iter, eps, S0, seed,
+                                   S_out, static_cast<hipStream_t>(stream));
+    });
+}
+
 bnmf_status bnmf_bld_add_f64(const double* X, size_t x, size_t y, size_t z, const double* alpha,
                              const double* beta, size_t max_iter, double eps, const double* S0,
                              uint64_t seed, double* S_out) {

@@ -70,6 +70,32 @@ bnmf_status bnmf_bld_mult_f32(const float* X, size_t x, size_t y, size_t z,
                               const float* S0, uint64_t seed, float* S_out);
 
 /* ------------------------------------------------------------------------
+ * No-allocation variant of bnmf_bld_mult_f32. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and the same kernels as bnmf_bld_mult_f32 (S
+ * is bitwise identical), but X (x*y), alpha (x), beta (z), S0 (x*y*z, or
+ * NULL: drawn from `seed` as bnmf_bld_mult_f32 draws it) and S_out (x*y*z)
+ * are device buffers the caller owns (X, S0 and S_out 16-byte aligned), and
+ * the solve is enqueued on `stream` (a hipStream_t; NULL: the default
+ * stream) without waiting. The workspace (digamma rows, partial axis sums,
+ * descriptors, work lists) is a fixed 32 MiB device array inside the
+ * library, held by each GPU it runs on until the process ends; an instance
+ * that needs more returns BNMF_ERR_INVALID_ARGUMENT (the partial axis sums
+ * dominate: about x*y*z/16 + 4*y*z*ceil(x/128) bytes, so 2048 x 2048 x 64
+ * needs about 25 MiB and fits). Nothing is
+ * allocated: no hipMalloc, hipFree or allocating library call. Every call
+ * on a GPU shares that array and leaves its contents behind; calls that
+ * overlap on one GPU (other streams or threads) race. X is not checked on
+ * the host (it is device memory): it must be nonnegative, as
+ * check_bld_params requires (src/bld/util_details.hpp:20-28).
+ * ---------------------------------------------------------------------- */
+bnmf_status bnmf_bld_mult_f32_noalloc(const float* X, size_t x, size_t y, size_t z,
+                                      const double* alpha, const double* beta, size_t max_iter,
+                                      int use_psi_appr, double eps, const float* S0, uint64_t seed,
+                                      float* S_out, void* stream);
+
+/* ------------------------------------------------------------------------
  * bld::bld_add (src/bld/bld_add.hpp:69-184): additive gradient ascent on
  * Z = log S with S = X * softmax_k(Z), step 0.1 / (iteration + 1)^0.55.
  * Same arguments as bnmf_bld_mult_*; every digamma is psi_appr, as in the
