@@ -440,8 +440,12 @@ __global__ void __launch_bounds__(64 * WAVES)
                 for (int m = 0; m < NCH; ++m) {
                     if ((uint32_t)m < nchunk) {
                         const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                        // RW-wide writes in the chunk visit order: lanes of
+                        // one access cycle on different banks
+                        VR cv;
 #pragma unroll
-                        for (int v = 0; v < RW; ++v) comb_self[lane * z + mm + v] = acc[m * RW + v];
+                        for (int v = 0; v < RW; ++v) vset<T, RW>(cv, v, acc[m * RW + v]);
+                        *reinterpret_cast<VR*>(comb_self + lane * z + mm) = cv;
                     }
                 }
             } else {

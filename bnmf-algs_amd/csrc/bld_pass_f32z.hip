@@ -572,11 +572,16 @@ __global__ void __launch_bounds__(64 * WAVES)
         __syncthreads();
         float* const comb_self = reinterpret_cast<float*>(raw_base + w * Lds::raw_stride);
         if (lane < nj) {
+            // RW-wide writes in the fiber's chunk visit order: lanes of one
+            // access cycle land on different banks (scalar writes at stride
+            // Z put every lane on one bank when Z is a multiple of 8)
 #pragma unroll
             for (int m = 0; m < NCH; ++m) {
                 const uint32_t mm = ((uint32_t)m ^ hsw) * RW;
+                VR cv;
 #pragma unroll
-                for (int v = 0; v < RW; ++v) comb_self[lane * Z + mm + v] = acc[m * RW + v];
+                for (int v = 0; v < RW; ++v) vset<float, RW>(cv, v, acc[m * RW + v]);
+                *reinterpret_cast<VR*>(comb_self + lane * Z + mm) = cv;
             }
         }
         __syncthreads();

@@ -419,12 +419,16 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
         __syncthreads();
         double* const comb_self = reinterpret_cast<double*>(raw_base + w * Lds::raw_stride);
         if (f < nj) {
+            // RW-wide writes in the chunk visit order (bank-spread, as the
+            // reads of the row are)
 #pragma unroll
             for (int u = 0; u < NH; ++u) {
                 if ((uint32_t)u < mcount) {
                     const uint32_t mm = (m0 + ((uint32_t)u ^ xm)) * RW;
+                    VR cv;
 #pragma unroll
-                    for (int v = 0; v < RW; ++v) comb_self[f * z + mm + v] = acc[u * RW + v];
+                    for (int v = 0; v < RW; ++v) vset<double, RW>(cv, v, acc[u * RW + v]);
+                    *reinterpret_cast<VR*>(comb_self + f * z + mm) = cv;
                 }
             }
         }
