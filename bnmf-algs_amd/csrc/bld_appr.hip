@@ -15,7 +15,8 @@
 //                reduced over i: column partials)
 //   factor       nu *= nom / (den + eps), rows renormalised; or mu, columns
 // Tiles are ti_rows x 64 fibers as in the update pass; lane f owns column
-// j0 + f. The double sums over c of update_nu / update_mu factor per fiber:
+// j0 + f (fibers deeper than 64 take a wavefront per fiber instead). The
+// double sums over c of update_nu / update_mu factor per fiber:
 // sum_c nu_ic mu_cj gm_ic = A_ij and sum_c nu_ic mu_cj gp_ijc = B_ij, so
 //   nom_ik = sum_j oa_ij mu_kj gp_ijk + oa2_ij mu_kj A_ij
 //   den_ik = sum_j oa_ij mu_kj gm_ik  + oa2_ij mu_kj B_ij
@@ -267,6 +268,110 @@ __global__ void __launch_bounds__(64 * WAVES)
     }
 }
 
+// Fibers deeper than 64 (z > 64): one wavefront per block walks a tile of
+// ti_rows x kWideCols fibers one fiber at a time, lanes over k in steps of
+// 64, and the sums over k (X_hat, A, B) are wavefront sums. The per-tile row
+// partials (over the tile's columns) and column partials (over its rows) are
+// accumulated in their own slots of the partial arrays by the lane that owns
+// k, a store at the tile's first column (row) and read-add-store after, so
+// each slot has one writer and a fixed order. The step modes recompute the
+// fiber's S and grad_plus in a second walk over k once A and B are known.
+template <typename T, typename ACC, int MODE, int PSI>
+__global__ void __launch_bounds__(64)
+    bld_appr_wide_kernel(const ApprInst* __restrict__ insts, const PassItem* __restrict__ items,
+                         const uint32_t* __restrict__ n_items_p) {
+    const uint32_t n_items = *n_items_p;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const PassItem item = items[it];
+        const ApprInst* __restrict__ I = insts + item.inst;
+        const uint32_t x = I->x, y = I->y, z = I->z;
+        const uint32_t i0 = item.ti * I->ti_rows, i1 = min(x, i0 + I->ti_rows);
+        const uint32_t j0 = item.tj * kWideCols, j1 = min(y, j0 + kWideCols);
+        const T epsT = static_cast<T>(I->eps);
+        const BNMF_GLOBAL T* __restrict__ nu = gconst<T>(I->nu);
+        const BNMF_GLOBAL T* __restrict__ mu = gconst<T>(I->mu);
+        const BNMF_GLOBAL T* __restrict__ psib = gconst<T>(I->psib);
+        const BNMF_GLOBAL T* __restrict__ psia = gconst<T>(I->psia);
+        const BNMF_GLOBAL T* __restrict__ psiA = gconst<T>(I->psiA);
+        // this tile's slots: rows [i*z + k] of tile column tj, columns
+        // [j*z + k] of tile row ti
+        BNMF_GLOBAL ACC* const rowp = gmut<ACC>(I->part_ipk) + (size_t)item.tj * x * z;
+        BNMF_GLOBAL ACC* const rowp2 = gmut<ACC>(I->part_ipk2) + (size_t)item.tj * x * z;
+        BNMF_GLOBAL ACC* const colp = gmut<ACC>(I->part_pjk) + (size_t)item.ti * y * z;
+        BNMF_GLOBAL ACC* const colp2 = gmut<ACC>(I->part_pjk2) + (size_t)item.ti * y * z;
+        for (uint32_t i = i0; i < i1; ++i) {
+            const size_t iz = (size_t)i * z;
+            for (uint32_t j = j0; j < j1; ++j) {
+                const size_t jz = (size_t)j * z;
+                const bool first_col = j == j0, first_row = i == i0;
+                // X_hat_ij = sum_k nu_ik mu_kj (update_X_hat :12-15)
+                T h = T(0);
+                for (uint32_t k = lane; k < z; k += 64u) h += nu[iz + k] * mu[(size_t)k * y + j];
+                h = wave_sum(h);
+                const T xv = gconst<T>(I->X)[(size_t)i * y + j];
+                T oa, oa2;
+                if constexpr (sizeof(T) == 8) {
+                    oa = xv / (h + epsT);
+                    oa2 = xv / (h * h + epsT);
+                } else {
+                    const bool live = xv > 0.0f;
+                    oa = live ? xv / (h + epsT) : 0.0f;
+                    oa2 = live ? xv / (h * h + epsT) : 0.0f;
+                }
+                if constexpr (MODE == APPR_SUMS) {
+                    // S_ijk (update_S :34-48) into both partials
+                    for (uint32_t k = lane; k < z; k += 64u) {
+                        const ACC s = ACC((oa * nu[iz + k]) * mu[(size_t)k * y + j]);
+                        rowp[iz + k] = first_col ? s : rowp[iz + k] + s;
+                        colp[jz + k] = first_row ? s : colp[jz + k] + s;
+                    }
+                } else {
+                    // grad_plus_k = psi(beta_eph_jk) - psi(S_ijk + 1),
+                    // A = sum_k nu mu gm_ik, B = sum_k nu mu gp_k
+                    auto grad_plus = [&](uint32_t k, T nk, T mk) {
+                        const T s = (oa * nk) * mk;
+                        T ps;
+                        if constexpr (sizeof(T) == 8)
+                            ps = psi_sel<PSI, T>(s + T(1));
+                        else
+                            ps = psi_ge1_f32(s + 1.0f);  // S >= 0
+                        return psib[jz + k] - ps;
+                    };
+                    T A = T(0), B = T(0);
+                    for (uint32_t k = lane; k < z; k += 64u) {
+                        const T nk = nu[iz + k], mk = mu[(size_t)k * y + j];
+                        const T gm = psiA[k] - psia[iz + k];
+                        const T nm = nk * mk;
+                        A += nm * gm;
+                        B += nm * grad_plus(k, nk, mk);
+                    }
+                    A = wave_sum(A);
+                    B = wave_sum(B);
+                    for (uint32_t k = lane; k < z; k += 64u) {
+                        const T nk = nu[iz + k], mk = mu[(size_t)k * y + j];
+                        const T gm = psiA[k] - psia[iz + k];
+                        const T g = grad_plus(k, nk, mk);
+                        if constexpr (MODE == APPR_NU) {
+                            // update_nu's numerator and denominator over j
+                            const ACC n = ACC((oa * mk) * g + (oa2 * mk) * A);
+                            const ACC d = ACC((oa * mk) * gm + (oa2 * mk) * B);
+                            rowp[iz + k] = first_col ? n : rowp[iz + k] + n;
+                            rowp2[iz + k] = first_col ? d : rowp2[iz + k] + d;
+                        } else {
+                            // update_mu's numerator and denominator over i
+                            const ACC n = ACC((oa * nk) * g + (oa2 * nk) * A);
+                            const ACC d = ACC((oa * nk) * gm + (oa2 * nk) * B);
+                            colp[jz + k] = first_row ? n : colp[jz + k] + n;
+                            colp2[jz + k] = first_row ? d : colp2[jz + k] + d;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // update_nu's step and row normalisation (:155-163) or update_mu's step and
 // column normalisation (:217-225). One wavefront per row i (HALF 0) or
 // column j (HALF 1), lane k: the per-tile partials of the numerator and
@@ -296,6 +401,37 @@ __global__ void __launch_bounds__(256)
         const BNMF_GLOBAL T* __restrict__ Pd = gconst<T>(HALF == 0 ? I->part_ipk2 : I->part_pjk2);
         const uint32_t nt = HALF == 0 ? I->n_tj : I->n_ti;
         const size_t tstride = (size_t)n_rows * z;
+        if (z > 64u) {
+            // deep rows: lanes step k by 64, the new values go to F and their
+            // sum over k is a wavefront sum; F is then divided in place (each
+            // lane rereads only what it wrote)
+            if (r < n_rows) {
+                T part = T(0);
+                for (uint32_t k = lane; k < z; k += 64u) {
+                    const size_t base = (size_t)r * z + k;
+                    const size_t fk = HALF == 0 ? base : (size_t)k * y + r;
+                    SumAcc<T> n, d;
+                    for (uint32_t t = 0; t < nt; ++t) {
+                        n.add(Pn[t * tstride + base]);
+                        d.add(Pd[t * tstride + base]);
+                    }
+                    T v;
+                    if constexpr (sizeof(T) == 8)
+                        v = F[fk] * (n.get() / (d.get() + epsT));
+                    else
+                        v = n.get() != 0.0f ? F[fk] * (n.get() / (d.get() + epsT)) : 0.0f;
+                    F[fk] = v;
+                    part += v;
+                }
+                T rs = wave_sum(part) + epsT;
+                if constexpr (sizeof(T) == 4) rs = rs != 0.0f ? rs : 1.0f;
+                for (uint32_t k = lane; k < z; k += 64u) {
+                    const size_t fk = HALF == 0 ? (size_t)r * z + k : (size_t)k * y + r;
+                    F[fk] = F[fk] / rs;
+                }
+            }
+            continue;
+        }
         const bool mine = r < n_rows && lane < z;
         const size_t fi = HALF == 0 ? (size_t)r * z + lane : (size_t)lane * y + r;
         if (mine) {
@@ -366,6 +502,9 @@ template <typename T, typename ACC, int MODE, int PSI>
 static hipError_t launch_appr_zb(int zb, dim3 grid, hipStream_t st, const ApprInst* insts,
                                  const PassItem* items, const uint32_t* n, uint32_t zmax) {
     switch (zb) {
+        case 0:  // z > 64
+            hipLaunchKernelGGL((bld_appr_wide_kernel<T, ACC, MODE, PSI>), grid, dim3(64), 0, st, insts, items, n);
+            return hipGetLastError();
         case 8: return launch_appr_t<T, ACC, 8, MODE, PSI>(grid, st, insts, items, n, zmax);
         case 16: return launch_appr_t<T, ACC, 16, MODE, PSI>(grid, st, insts, items, n, zmax);
         case 32: return launch_appr_t<T, ACC, 32, MODE, PSI>(grid, st, insts, items, n, zmax);

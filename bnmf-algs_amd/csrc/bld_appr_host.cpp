@@ -52,15 +52,16 @@ ApprBatch::ApprBatch(int dtype, double eps, const std::vector<ApprSpec>& specs)
     // layouts
     size_t total = 0;
     for (const ApprSpec& s : specs) {
-        // the approximate solver's passes keep a tile's factors per column
-        // in LDS and registers sized for fibers of at most 64
-        BNMF_REQUIRE(s.z >= 1 && s.z <= kFusedZ, "bld_appr supports z in [1, 64]");
+        BNMF_REQUIRE(s.z >= 1, "bld_appr needs z >= 1");
         L_ l{};
         l.x = s.x;
         l.y = s.y;
         l.z = s.z;
-        l.n_tj = ceil_div(s.y, kTileCols);
-        l.ti_rows = appr_ti_rows(s.x, l.n_tj);
+        // fibers deeper than 64 take the wavefront-per-fiber pass on
+        // kWideRows x kWideCols tiles
+        const bool deep = s.z > kFusedZ;
+        l.n_tj = ceil_div(s.y, deep ? kWideCols : kTileCols);
+        l.ti_rows = deep ? kWideRows : appr_ti_rows(s.x, l.n_tj);
         l.n_ti = ceil_div(s.x, l.ti_rows);
         l.nA = ceil_div(s.x, kPrepRows);
         l.n_cj = ceil_div(s.y, kPrepCols);

@@ -37,14 +37,6 @@
 
 namespace bnmf {
 
-// sum over the wavefront, the same value in every lane
-template <typename A>
-__device__ __forceinline__ A wave_sum(A v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
 template <typename T, typename ACC, int MODE, int PSI>
 __global__ void __launch_bounds__(64)
     bld_wide_kernel(const BldInst* __restrict__ insts, const PassItem* __restrict__ items,

@@ -102,7 +102,6 @@ void appr_one(int dtype, const T* X, size_t x, size_t y, size_t z, const double*
     check_params(X, x, y, z, alpha, beta);
     // z = 0: nu, mu and S have no elements
     if (z == 0) return;
-    BNMF_REQUIRE(z <= bnmf::kFusedZ, "bld_appr supports z in [1, 64]");
     // an empty X (x*y = 0) runs no iteration; the factors come back as
     // initialised, as they do for an empty instance of a batch
     bnmf::ApprSpec s;

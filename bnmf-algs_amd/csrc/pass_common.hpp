@@ -33,6 +33,15 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// sum over the wavefront, the same value in every lane (a butterfly: every
+// lane adds the same pairs in the same order)
+template <typename A>
+__device__ __forceinline__ A wave_sum(A v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 template <typename A>
 struct Kahan {
     A s = A(0), c = A(0);

@@ -96,9 +96,6 @@ def test_sibling_solvers_report_invalid_argument(hip):
         # z = 0: an empty instance
         assert call(good, 0) == hip.OK, name
         assert call(good, 1 << 31) == hip.ERR_INVALID_ARGUMENT, name
-    # bld_appr's passes hold a tile's factors for z <= 64
-    assert calls["appr"](good, 65) == hip.ERR_INVALID_ARGUMENT
-    assert b"z in [1, 64]" in lib.bnmf_last_error()
     # null X with a nonempty shape
     st = lib.bnmf_bld_add_f64(None, 1, 2, 1, A, A, 5, 1e-50, None, 0, O)
     assert st == hip.ERR_INVALID_ARGUMENT and b"X is null" in lib.bnmf_last_error()

@@ -9,8 +9,12 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def restore_table(hip):
+    # the test starts from an empty table (the library loads the shipped one)
+    # and puts the loaded entries back afterwards
     snap = {(dt, z, f): hip.tune_get(dt, z, f) for dt in (np.float32, np.float64)
             for z in hip.TUNE_Z_CLASSES for f in (1 << 12, 1 << 15, 1 << 17, 1 << 19, 1 << 21)}
+    for dt, z, f in snap:
+        hip.tune_set(dt, z, f, 0)
     yield hip
     for (dt, z, f), rows in snap.items():
         hip.tune_set(dt, z, f, rows)

@@ -3,11 +3,12 @@ k). The reference bounds z nowhere (check_bld_params,
 src/bld/util_details.hpp:20-28), so a caller with n_components = 100 must get
 the same solver: bld_mult (FP64 and FP32), bld_add, log_marginal_S,
 tensor_sums and bld_fact against the FP64 oracle at z in {65, 100, 128, 257},
-deep and shallow instances batched together equal to each solved alone, and a
-deep request through the server.
+bld_appr at z in {65, 100, 130}, deep and shallow instances batched together
+equal to each solved alone, and a deep request through the server.
 
-Tolerances: FP64 forms 1e-10 relative (normwise) on S and the log-marginal;
-FP32 bulk 1e-4 of the FP64 oracle; sums and scoring 1e-12.
+Tolerances: FP64 forms 1e-10 relative (normwise) on S and the log-marginal
+(bld_appr after one iteration 1e-12, as for shallow fibers); FP32 bulk 1e-4
+of the FP64 oracle (bld_appr 5e-5); sums and scoring 1e-12.
 """
 import numpy as np
 import pytest
@@ -146,3 +147,61 @@ def test_empty_depth(hip):
     p = hip.Params(1.0, 1.0, [1.0] * 3, [])
     S = hip.bld_mult(X, 0, p, 5)
     assert S.shape == (3, 4, 0)
+
+
+class _ApprInst:
+    def __init__(self, X, z, params, nu0, mu0):
+        self.X, self.z, self.params, self.nu0, self.mu0, self.seed = X, z, params, nu0, mu0, 0
+
+
+def _appr_problem(hip, x, y, z, seed):
+    p = hip.Params(40.0, 1.0, [1.0] * x, [1.0] * z)
+    X = hip.sample_X(seed, (x, y, z), p)
+    rng = np.random.default_rng(seed)
+    nu0 = rng.dirichlet(np.ones(z), size=x)
+    mu0 = rng.dirichlet(np.ones(z), size=y).T.copy()
+    return p, X, nu0, mu0
+
+
+APPR_DEEP = [(40, 70, 65), (33, 50, 100), (20, 37, 130)]
+
+
+@pytest.mark.parametrize("x,y,z", APPR_DEEP)
+def test_appr_fp64_deep_matches_oracle(hip, oracle, x, y, z):
+    p, X, nu0, mu0 = _appr_problem(hip, x, y, z, z + 1)
+    print(f"[deep] bld_appr FP64 {x}x{y}x{z}", flush=True)
+    S, nu, mu = hip.bld_appr(X, z, p, 1, nu0=nu0, mu0=mu0)
+    Sr, nur, mur = oracle.bld_appr(X, z, p.alpha, p.beta, nu0, mu0, max_iter=1)
+    assert _rel(nu, nur) < 1e-12 and _rel(mu, mur) < 1e-12 and _rel(S, Sr) < 1e-12
+    S, nu, mu = hip.bld_appr(X, z, p, 30, nu0=nu0, mu0=mu0)
+    Sr, nur, mur = oracle.bld_appr(X, z, p.alpha, p.beta, nu0, mu0, max_iter=30)
+    assert _rel(nu, nur) < 1e-10 and _rel(mu, mur) < 1e-10 and _rel(S, Sr) < 1e-10
+    # rows of nu and columns of mu stay on the simplex, S_ij+ = X
+    assert np.abs(nu.sum(1) - 1).max() < 1e-12 and np.abs(mu.sum(0) - 1).max() < 1e-12
+    assert np.abs(S.sum(2) - X).max() <= 1e-12 * max(X.max(), 1)
+
+
+def test_appr_fp32_deep_close_to_fp64_oracle(hip, oracle):
+    x, y, z = 41, 50, 100
+    p, X, nu0, mu0 = _appr_problem(hip, x, y, z, 9)
+    nu0f, mu0f = nu0.astype(np.float32), mu0.astype(np.float32)
+    S, _, _ = hip.bld_appr(X.astype(np.float32), z, p, 60, nu0=nu0f, mu0=mu0f, dtype=np.float32)
+    Sr, _, _ = oracle.bld_appr(X, z, p.alpha, p.beta, nu0f.astype(np.float64),
+                               mu0f.astype(np.float64), max_iter=60)
+    assert _rel(S, Sr) < 5e-5, _rel(S, Sr)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_appr_deep_and_shallow_batched_equal_alone(hip, dtype):
+    shapes = [(30, 45, 100), (25, 70, 17), (20, 33, 65)]
+    insts, alone = [], []
+    for n, (x, y, z) in enumerate(shapes):
+        p, X, nu0, mu0 = _appr_problem(hip, x, y, z, 50 + n)
+        X, nu0, mu0 = X.astype(dtype), nu0.astype(dtype), mu0.astype(dtype)
+        insts.append(_ApprInst(X, z, p, nu0, mu0))
+        alone.append(hip.bld_appr(X, z, p, 10, nu0=nu0, mu0=mu0, dtype=dtype))
+    with hip.ApprBatch(insts, dtype=dtype) as b:
+        b.solve(10)
+        for n in range(len(shapes)):
+            for g, a in zip(b.get(n), alone[n]):
+                assert np.array_equal(g, a), shapes[n]
