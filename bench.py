@@ -254,6 +254,26 @@ def measure_add(hip, c3f, elems3, fibers3, iters3):
         "workload": "the headline's 256 instances, 200 iterations of bld::bld_add"}
 
 
+def measure_appr(hip, c3f, elems3, iters3):
+    """bld_appr (FP32) on the headline's 256 instances: one timed solve of
+    iters3 iterations and the graph-timed iteration."""
+    log("[bench] batched FP32 bld_appr on the same instances")
+    bp = hip.ApprBatch(c3f, dtype=np.float32)
+    bp.solve(2)
+    t0 = time.perf_counter()
+    bp.solve(iters3)
+    tp = time.perf_counter() - t0
+    ip = bp.time_iteration(10)
+    bp.close()
+    log(f"[bench] bld_appr FP32: {tp*1e3:.1f} ms/solve, iteration {ip*1e3:.1f} us")
+    return {"value": elems3 * iters3 / tp, "unit": "S-element updates/s",
+            "ms_per_solve": tp * 1e3, "iter_us": ip * 1e3,
+            "bound": "latency and VALU (S is formed from the factors in registers; HBM traffic "
+                     "is X and the factors only)",
+            "workload": "the headline's 256 instances, 200 iterations of bld::bld_appr, "
+                        "factors drawn from Dirichlet(1)"}
+
+
 def measure_c1(hip, iters=500, solves=1, reps=20):
     """One 1024x1024x32 instance in the plain FP64 form (exact digamma):
     solve time and the graph-timed update pass (algorithmic bytes 2*x*y*z*8
@@ -362,6 +382,8 @@ def main():
     ap.add_argument("--no-appr", action="store_true", help="skip the bld_appr measurement")
     ap.add_argument("--add-only", action="store_true",
                     help="only the batched FP32 bld_add measurement (e.g. for profiling its pass)")
+    ap.add_argument("--appr-only", action="store_true",
+                    help="only the batched FP32 bld_appr measurement (e.g. for profiling its passes)")
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--only", choices=("c1", "c2"), default=None,
                     help="only one single-instance configuration (e.g. for profiling its pass): "
@@ -443,6 +465,9 @@ def main():
         print(json.dumps({"bld_add_batched_fp32": measure_add(hip, c3f, elems3, fibers3, iters3)}),
               flush=True)
         return
+    if args.appr_only:
+        print(json.dumps({"bld_appr_batched_fp32": measure_appr(hip, c3f, elems3, iters3)}), flush=True)
+        return
     b32 = hip.Batch(c3f, dtype=np.float32, use_psi_appr=True)
     log("[bench] timing batched FP32 solves")
     t_step = timed_solves(b32, iters3, args.steps, args.warmup)
@@ -508,22 +533,7 @@ def main():
 
     # ----------------------- sibling solver: bld_appr on the same 256 instances
     if not args.only_headline and not args.no_appr:
-        log("[bench] batched FP32 bld_appr on the same instances")
-        bp = hip.ApprBatch(c3f, dtype=np.float32)
-        bp.solve(2)
-        t0 = time.perf_counter()
-        bp.solve(iters3)
-        tp = time.perf_counter() - t0
-        ip = bp.time_iteration(10)
-        out["bld_appr_batched_fp32"] = {
-            "value": elems3 * iters3 / tp, "unit": "S-element updates/s",
-            "ms_per_solve": tp * 1e3, "iter_us": ip * 1e3,
-            "bound": "latency and VALU (S is formed from the factors in registers; HBM traffic "
-                     "is X and the factors only)",
-            "workload": "the headline's 256 instances, 200 iterations of bld::bld_appr, "
-                        "factors drawn from Dirichlet(1)"}
-        bp.close()
-        log(f"[bench] bld_appr FP32: {tp*1e3:.1f} ms/solve, iteration {ip*1e3:.1f} us")
+        out["bld_appr_batched_fp32"] = measure_appr(hip, c3f, elems3, iters3)
 
     # ------------------------------------------- configs 1 (FP64) and 2 (FP32)
     if not args.only_headline:

@@ -277,8 +277,10 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
                             for (int v = 0; v < RW; ++v) {
                                 const double sk = vget<double, RW>(sv, v);
-                                const double sn = as_stored(sk * ((vget<double, RW>(gw, v) + nom) * x_over_s /
-                                                                  (vget<double, RW>(gv, v) + den + eps)));
+                                // gm + den + eps is in div_moderate's range
+                                // unless gm + den cancels eps below 2^-1000
+                                const double sn = as_stored(sk * div_moderate((vget<double, RW>(gw, v) + nom) * x_over_s,
+                                                                               vget<double, RW>(gv, v) + den + eps));
                                 vset<double, RW>(sv, v, sn);
                                 acc[u * RW + v] += sn;
                             }
