@@ -26,17 +26,26 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-// Rows per tile: enough tiles to give one instance's launch ~`target`
-// blocks, between 4 and 64 rows. Measured on MI355X (1024x1024x32): 512
-// blocks per iteration beat 2048 and 8192 (more tiles add column partials
-// that the factor and prep kernels must reduce). BNMF_APPR_BLOCKS overrides
-// the target for experiments.
-uint32_t appr_ti_rows(uint32_t x, uint32_t n_tj) {
-    static const uint32_t target = [] {
+// Rows per tile: enough tiles to give an instance's launch about one block
+// per 2048 fibers, between 16 and 512 blocks, with tiles of 4 to 64 rows.
+// The plan depends on the shape only, so an instance gives the same bits
+// alone and batched. Measured on MI355X: 1024x1024x32 alone, 512 blocks
+// (32-row tiles) 314 us per iteration against 518 us at 256 or fewer, and
+// 2048 or 8192 blocks slower again (more tiles add column partials that the
+// factor and prep kernels must reduce); the 256-instance batch (x, y up to
+// 512), 3524 us per iteration at 512 blocks per instance, 2706 us at 128,
+// 2566 us at 16 (the factor kernel's partial sums over row tiles dominate
+// once every instance has many short tiles). BNMF_APPR_BLOCKS sets a fixed
+// target for every instance instead, for experiments.
+uint32_t appr_ti_rows(uint32_t x, uint32_t y, uint32_t n_tj) {
+    static const uint32_t fixed_target = [] {
         const char* e = std::getenv("BNMF_APPR_BLOCKS");
         const long v = e ? std::strtol(e, nullptr, 10) : 0L;
-        return v > 0 ? (uint32_t)std::min<long>(v, 1 << 20) : 512u;
+        return v > 0 ? (uint32_t)std::min<long>(v, 1 << 20) : 0u;
     }();
+    const uint64_t by_size = ((uint64_t)x * y) / 2048u;
+    const uint32_t target =
+        fixed_target ? fixed_target : (uint32_t)std::min<uint64_t>(std::max<uint64_t>(by_size, 16u), 512u);
     const uint32_t want = std::max<uint32_t>(1, target / std::max<uint32_t>(n_tj, 1));
     uint32_t ti = ceil_div(x, want);
     ti = (ti + 3) / 4 * 4;
@@ -61,7 +70,7 @@ ApprBatch::ApprBatch(int dtype, double eps, const std::vector<ApprSpec>& specs)
         // kWideRows x kWideCols tiles
         const bool deep = s.z > kFusedZ;
         l.n_tj = ceil_div(s.y, deep ? kWideCols : kTileCols);
-        l.ti_rows = deep ? kWideRows : appr_ti_rows(s.x, l.n_tj);
+        l.ti_rows = deep ? kWideRows : appr_ti_rows(s.x, s.y, l.n_tj);
         l.n_ti = ceil_div(s.x, l.ti_rows);
         l.nA = ceil_div(s.x, kPrepRows);
         l.n_cj = ceil_div(s.y, kPrepCols);
