@@ -318,6 +318,18 @@ def measure_c2(hip, iters=300, reps=20, accuracy=True):
         res = {"rel_err_S_vs_fp64": rel_err(S2, S2d),
                "log_marginal_gap_vs_fp64": abs(lm2 - lm2d) / abs(lm2d)}
         del S2, S2d
+    if accuracy:
+        # the same solve through the plain call, which takes host buffers:
+        # uploads X and S0, solves, downloads S (PCIe-inclusive; one call
+        # after a warm-up call)
+        c2f = as_dtype(c2, hip, np.float32)
+        hip.bld_mult(c2f.X, 64, c2f.params, 2, True, S0=c2f.S0, dtype=np.float32)
+        t0 = time.perf_counter()
+        hip.bld_mult(c2f.X, 64, c2f.params, iters, True, S0=c2f.S0, dtype=np.float32)
+        th = time.perf_counter() - t0
+        res["host_buffers"] = {"ms_per_solve": th * 1e3, "updates_per_s": 2048 * 2048 * 64 * iters / th,
+                               "mode": "bnmf_bld_mult_f32 on host X and S0, S returned to the host"}
+        del c2f
     e2 = 2048 * 2048 * 64
     alg = 2 * e2 * 4 + 2048 * 2048 * 4
     res = {"value": e2 * iters / t2, "unit": "S-element updates/s", "ms_per_solve": t2 * 1e3,
