@@ -76,3 +76,29 @@ def test_empty_instances_plan(table):
         assert hip.tile_rows(dt, 7, 0, 3) == 0
         assert hip.tile_rows(dt, 0, 0, 1) == 0
         assert hip.tile_rows(dt, 7, 5, 3) >= 1
+
+
+def test_shipped_table(table):
+    """The library loads tune/gfx950.txt by default (checked in a fresh
+    process: the table is global); the large single-instance classes carry
+    the heights measured for them (2048x2048x64 and x32: 128 rows, FP64
+    1024x1024x32: 16 rows) and the headline classes keep 32."""
+    import os
+    import subprocess
+    import sys
+    hip = table
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = os.path.join(root, "bnmf-algs_amd", "tune", "gfx950.txt")
+    env = {k: v for k, v in os.environ.items() if k != "BNMF_TUNE_FILE"}
+    src = subprocess.run([sys.executable, "-c", "import bnmf_hip; print(bnmf_hip.tune_source())"],
+                         cwd=os.path.join(root, "bnmf-algs_amd"), env=env, capture_output=True,
+                         text=True, timeout=120).stdout.strip()
+    assert os.path.realpath(src) == os.path.realpath(path), src
+    rows = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    assert rows and all(r[0] in ("f32", "f64") and int(r[3]) % 4 == 0 and 4 <= int(r[3]) <= 256
+                        for r in rows)
+    hip.tune_load(path)
+    assert hip.tile_rows(np.float32, 2048, 2048, 64) == 128
+    assert hip.tile_rows(np.float32, 2048, 2048, 32) == 128
+    assert hip.tile_rows(np.float64, 1024, 1024, 32) == 16
+    assert hip.tune_get(np.float32, 27, 300 * 300) == 32
