@@ -398,9 +398,14 @@ Batch::Batch(int dtype, int psi_mode, double eps, const std::vector<InstSpec>& s
         bk.lane = 0;
         groups_[0].push_back(bk);
     }
-    // up to four streams; each group goes to the least loaded one, largest
-    // work first
-    const int ns = (int)std::min<size_t>(4, std::max(groups_[0].size(), groups_[1].size()));
+    // up to four streams (BNMF_LANES: 1 to 16); each group goes to the least
+    // loaded one, largest work first
+    static const size_t max_lanes = [] {
+        const char* e = std::getenv("BNMF_LANES");
+        const long v = e ? std::strtol(e, nullptr, 10) : 0L;
+        return v > 0 ? (size_t)std::min<long>(v, 16) : (size_t)4;
+    }();
+    const int ns = (int)std::min<size_t>(max_lanes, std::max(groups_[0].size(), groups_[1].size()));
     if (ns > 1) {
         for (int g = 0; g < 2; ++g) {
             std::vector<Bucket>& gs = groups_[g];
