@@ -232,7 +232,6 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                             }
                             *reinterpret_cast<VR*>(gb + mm) = gw;
                         }
-                        __builtin_amdgcn_sched_barrier(0);
                     }
                     if constexpr (PSI == PSI_F64_EXACT) {
                         if (__builtin_amdgcn_ballot_w64(slow) != 0) {
@@ -286,7 +285,6 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                             }
                             *reinterpret_cast<VR*>(fb + mm) = sv;
                         }
-                        __builtin_amdgcn_sched_barrier(0);
                     }
                 } else {
                     // bld_add (src/bld/bld_add.hpp:137-181): grad_S =
@@ -313,7 +311,6 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                             }
                             *reinterpret_cast<VR*>(gb + mm) = gw;
                         }
-                        __builtin_amdgcn_sched_barrier(0);
                     }
 #pragma unroll
                     for (int o = 1; o < LPF; o <<= 1) smult += __shfl_xor(smult, o);
@@ -338,7 +335,6 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                             }
                             *reinterpret_cast<VR*>(gb + mm) = gw;
                         }
-                        __builtin_amdgcn_sched_barrier(0);
                     }
 #pragma unroll
                     for (int o = 1; o < LPF; o <<= 1) norm += __shfl_xor(norm, o);
