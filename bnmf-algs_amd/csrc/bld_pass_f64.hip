@@ -201,8 +201,10 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                 if constexpr (MODE == PASS_UPDATE) {
                     const double xr = 1.0 / (xv + eps);
                     // Phase A: g = psi(beta_eph) - psi(S+1) into the gradient
-                    // row, S_ij+ and nom (sums in the lane's visit order)
-                    double sumS = 0.0, nom = 0.0;
+                    // row, S_ij+, nom and denom (sums in the lane's visit
+                    // order; an element whose S+1 is outside [1, 10) adds its
+                    // denom term in the rare-path loop below instead)
+                    double sumS = 0.0, nom = 0.0, den = 0.0;
                     bool slow = false;
 #pragma unroll
                     for (int u = 0; u < NH; ++u) {
@@ -216,19 +218,22 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                             for (int v = 0; v < RW; ++v) {
                                 const double sk = vget<double, RW>(sv, v);
                                 double ps;
+                                bool fast = true;
                                 if constexpr (PSI == PSI_F64_EXACT) {
                                     // 1 <= S+1 < 10 inline; other arguments
                                     // are redone below (a rare path)
                                     const double xa = sk + 1.0;
-                                    const bool fast = xa >= 1.0 && xa < 10.0;
+                                    fast = xa >= 1.0 && xa < 10.0;
                                     slow = slow || !fast;
                                     ps = psi_1to10(fast ? xa : 1.0);
                                 } else {
                                     ps = psi_sp1<PSI, double>(sk);
                                 }
-                                vset<double, RW>(gw, v, vget<double, RW>(pb, v) - ps);
+                                const double g = vget<double, RW>(pb, v) - ps;
+                                vset<double, RW>(gw, v, g);
                                 sumS += sk;
                                 nom += sk * xr * vget<double, RW>(gv, v);
+                                if (fast) den += sk * xr * g;
                             }
                             *reinterpret_cast<VR*>(gb + mm) = gw;
                         }
@@ -239,21 +244,13 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
                                 const uint32_t mm = (m0 + (u ^ xm)) * RW;
                                 for (int v = 0; v < RW; ++v) {
                                     const double xa = fb[mm + v] + 1.0;
-                                    if (!(xa >= 1.0 && xa < 10.0)) gb[mm + v] = pbb[mm + v] - psi_f64_general(xa);
+                                    if (!(xa >= 1.0 && xa < 10.0)) {
+                                        const double g = pbb[mm + v] - psi_f64_general(xa);
+                                        gb[mm + v] = g;
+                                        den += fb[mm + v] * xr * g;
+                                    }
                                 }
                             }
-                        }
-                    }
-                    // denom in the same visit order
-                    double den = 0.0;
-#pragma unroll
-                    for (int u = 0; u < NH; ++u) {
-                        if ((uint32_t)u < mcount) {
-                            const uint32_t mm = (m0 + ((uint32_t)u ^ xm)) * RW;
-                            const VR sv = *reinterpret_cast<const VR*>(fb + mm);
-                            const VR gw = *reinterpret_cast<const VR*>(gb + mm);
-#pragma unroll
-                            for (int v = 0; v < RW; ++v) den += vget<double, RW>(sv, v) * xr * vget<double, RW>(gw, v);
                         }
                     }
                     // the other parts of the fiber (each butterfly step adds a
