@@ -13,8 +13,8 @@ benchmark configurations touch, chosen by timing on the GPU this runs on.
 3. FP32, the serving workload's requests (x, y log-uniform in [64, 2048],
    z in [8, 64]), each size class not set by steps 1-2 timed on its own
    requests batched together, heights 32, 64, 128.
-4. FP64, 1024 x 1024 x 32 with exact digamma (the plain form), heights 16,
-   32, 64, 128.
+4. FP64, 1024 x 1024 x 32 with exact digamma (the plain form), heights 12
+   to 32 in steps of 4, 64 and 128.
 Classes no step reaches keep the built-in rule.
 """
 import argparse
@@ -73,8 +73,8 @@ def main():
 
     log("[tune] 4: 1024x1024x32 (FP64)")
     c1 = bench.single(hip, 1024, 1024, 32, 40.0, 0xB1D1)
-    r = hip.autotune(np.float64, instances=[c1], candidates=(16, 32, 64, 128), reps=5, trials=2,
-                     use_psi_appr=False, log=log)
+    r = hip.autotune(np.float64, instances=[c1], candidates=(12, 16, 20, 24, 28, 32, 64, 128), reps=5,
+                     trials=2, use_psi_appr=False, log=log)
     for k in r:
         set_by[("f64",) + tuple(k)] = "1024x1024x32 FP64 alone"
 
