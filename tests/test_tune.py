@@ -102,3 +102,28 @@ def test_shipped_table(table):
     assert hip.tile_rows(np.float32, 2048, 2048, 32) == 128
     assert hip.tile_rows(np.float64, 1024, 1024, 32) == 28
     assert hip.tune_get(np.float32, 27, 300 * 300) == 32
+
+
+def test_large_lone_instances_keep_enough_tiles(table):
+    """A cap shortens tiles but never lengthens them past what keeps the GPU
+    filled: an instance of 2^20 fibers or more gets at least enough row tiles
+    for 512 tiles in all (FP32, 64-fiber columns) or 1024 (FP64, 32-fiber
+    columns at z <= 32), with tiles of at least 16 rows; smaller instances
+    get ceil(x / cap) tiles of equal height, a multiple of 4."""
+    hip = table
+    # FP32 4096 x 256 x 8: 4 column tiles, so 128 row tiles of 32 rows even
+    # under a 512-row cap
+    hip.tune_set(np.float32, 8, 4096 * 256, 512)
+    r = hip.tile_rows(np.float32, 4096, 256, 8)
+    assert r == 32 and -(-4096 // r) * (256 // 64) >= 512
+    # FP64 4096 x 256 x 8: 8 column tiles -> 128 row tiles of 32 rows
+    hip.tune_set(np.float64, 8, 4096 * 256, 512)
+    r = hip.tile_rows(np.float64, 4096, 256, 8)
+    assert r == 32 and -(-4096 // r) * (256 // 32) >= 1024
+    # a 1024-row cap on 1024 x 1024 FP32 (16 column tiles): 32 row tiles of
+    # 32 rows
+    hip.tune_set(np.float32, 8, 1 << 20, 1024)
+    assert hip.tile_rows(np.float32, 1024, 1024, 8) == 32
+    # below 2^20 fibers the cap alone decides: 1000 rows under 512 -> 2 x 500
+    hip.tune_set(np.float32, 8, 1000 * 10, 512)
+    assert hip.tile_rows(np.float32, 1000, 10, 8) == 500
