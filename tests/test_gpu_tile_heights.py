@@ -43,3 +43,24 @@ def test_odd_tile_heights_match_oracle(hip, oracle, rows, dtype, x, y, z, appr):
                           use_psi_appr=appr)
     tol = 1e-10 if dtype == np.float64 else 1e-4
     assert _rel(S, ref) <= tol, _rel(S, ref)
+
+
+@pytest.mark.parametrize("rows", [4, 12, 28, 40, 100])
+@pytest.mark.parametrize("dtype,x,y,z", [(np.float64, 130, 70, 17), (np.float32, 130, 70, 17),
+                                         (np.float32, 101, 66, 32)])
+def test_add_at_odd_tile_heights_matches_oracle(hip, oracle, rows, dtype, x, y, z):
+    # bld_add runs the same step pass in its additive mode (tolerances as for
+    # bld_add elsewhere: 1e-10 FP64, 1e-5 FP32)
+    iters = 40
+    p = hip.Params(40.0, 1.0, [1.0] * x, [1.0] * z)
+    X = hip.sample_X(5 * x + z, (x, y, z), p)
+    S0 = hip.init_S(5 * x + z + 1, X, z).astype(dtype)
+    old = hip.tune_get(dtype, z, x * y)
+    hip.tune_set(dtype, z, x * y, rows)
+    try:
+        S = hip.bld_add(X.astype(dtype), z, p, iters, S0=S0, dtype=dtype)
+    finally:
+        hip.tune_set(dtype, z, x * y, old)
+    ref = oracle.bld_add(X, z, p.alpha, p.beta, S0.astype(np.float64), max_iter=iters)
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    assert _rel(S, ref) <= tol, _rel(S, ref)
